@@ -1,0 +1,272 @@
+// extern "C" boundary of libnusht_amd.so (declared in include/nusht_amd.h).
+#include "../../include/nusht_amd.h"
+
+#include <exception>
+#include <memory>
+#include <new>
+
+#include "legendre.h"
+#include "nufft.h"
+#include "ring.h"
+#include "stages.h"
+
+using namespace nusht;
+
+struct nusht_plan_s {
+  NufftParams np;
+  int lmax = 0, spin = 0, ncomp = 1;
+  long npts_max = 0;
+  LegendreTables tab;
+  std::unique_ptr<NufftPlan> nu;
+  DevBuf coef, G, ptg;
+  int nmrows = 0;
+  static constexpr int kMaxMaps = 8;
+};
+
+struct nusht_ringplan_s {
+  std::unique_ptr<RingPlan> rp;
+};
+
+namespace {
+
+template <typename F>
+int guard(F&& f) {
+  try {
+    f();
+    return NUSHT_OK;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_error("host allocation failed");
+    return NUSHT_ERR_NOMEM;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return NUSHT_ERR_HIP;
+  }
+}
+
+inline hipStream_t S(nusht_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+void check_plan(nusht_plan_t p) { NUSHT_REQUIRE(p != nullptr, kErrArg, "null plan"); }
+
+}  // namespace
+
+extern "C" {
+
+const char* nusht_last_error(void) { return last_error(); }
+int nusht_version(void) { return 1; }
+
+// ------------------------------------------------------------------ stages
+int nusht_doubling_f64(const double* map, int ntheta, int nphi, int spin, double* dmap, nusht_stream_t s) {
+  return guard([&] { doubling<double>(map, ntheta, nphi, spin, dmap, S(s)); });
+}
+int nusht_doubling_f32(const float* map, int ntheta, int nphi, int spin, float* dmap, nusht_stream_t s) {
+  return guard([&] { doubling<float>(map, ntheta, nphi, spin, dmap, S(s)); });
+}
+int nusht_doubling_c128(const double* map, int ntheta, int nphi, int spin, double* dmap, nusht_stream_t s) {
+  return guard([&] {
+    doubling<double2>(reinterpret_cast<const double2*>(map), ntheta, nphi, spin, reinterpret_cast<double2*>(dmap), S(s));
+  });
+}
+int nusht_adjoint_doubling_f64(const double* dmap, int ntheta, int nphi, int spin, double* map, nusht_stream_t s) {
+  return guard([&] { adjoint_doubling<double>(dmap, ntheta, nphi, spin, map, S(s)); });
+}
+int nusht_adjoint_doubling_f32(const float* dmap, int ntheta, int nphi, int spin, float* map, nusht_stream_t s) {
+  return guard([&] { adjoint_doubling<float>(dmap, ntheta, nphi, spin, map, S(s)); });
+}
+int nusht_adjoint_doubling_c128(const double* dmap, int ntheta, int nphi, int spin, double* map, nusht_stream_t s) {
+  return guard([&] {
+    adjoint_doubling<double2>(reinterpret_cast<const double2*>(dmap), ntheta, nphi, spin,
+                              reinterpret_cast<double2*>(map), S(s));
+  });
+}
+int nusht_pointing_f64(const double* theta, const double* phi0, const int64_t* nph, const int64_t* ofs, int nring,
+                       int64_t max_nph, const double* d_theta, const double* d_phi, double* out_theta,
+                       double* out_phi, double* out_gamma, nusht_stream_t s) {
+  return guard([&] {
+    pointing(theta, phi0, reinterpret_cast<const long*>(nph), reinterpret_cast<const long*>(ofs), nring, max_nph,
+             d_theta, d_phi, out_theta, out_phi, out_gamma, S(s));
+  });
+}
+
+// ------------------------------------------------------------------ ring plan
+int nusht_ringplan_create(const nusht_ring_desc* d, nusht_stream_t s, nusht_ringplan_t* out) {
+  return guard([&] {
+    NUSHT_REQUIRE(d && out, kErrArg, "null argument");
+    NUSHT_REQUIRE(d->nring >= 1 && d->theta && d->phi0 && d->nph && d->ofs, kErrArg, "bad ring description");
+    NUSHT_REQUIRE(d->spin >= 0 && d->spin <= 2, kErrArg, "spin must be 0, 1 or 2");
+    std::vector<double> th(d->theta, d->theta + d->nring), p0(d->phi0, d->phi0 + d->nring);
+    std::vector<long> nph(d->nph, d->nph + d->nring), ofs(d->ofs, d->ofs + d->nring);
+    auto rp = std::make_unique<nusht_ringplan_s>();
+    rp->rp = std::make_unique<RingPlan>(d->lmax, d->spin, th, p0, nph, ofs, S(s));
+    NUSHT_HIP(hipStreamSynchronize(S(s)));
+    *out = rp.release();
+  });
+}
+
+int nusht_ringplan_destroy(nusht_ringplan_t p) {
+  return guard([&] { delete p; });
+}
+
+int64_t nusht_ringplan_npix(nusht_ringplan_t p) { return p ? p->rp->npix() : -1; }
+
+int nusht_ringplan_synthesis(nusht_ringplan_t p, const double* alm, int ncomp, double* map0, double* map1,
+                             nusht_stream_t s) {
+  return guard([&] {
+    NUSHT_REQUIRE(p, kErrArg, "null ring plan");
+    NUSHT_REQUIRE(p->rp->spin() == 0 || map1, kErrArg, "spin > 0 needs two output maps");
+    p->rp->synthesis(reinterpret_cast<const double2*>(alm), ncomp, map0, map1, S(s));
+  });
+}
+
+int nusht_ringplan_adjoint(nusht_ringplan_t p, const double* map0, const double* map1, double* alm, int ncomp,
+                           nusht_stream_t s) {
+  return guard([&] {
+    NUSHT_REQUIRE(p, kErrArg, "null ring plan");
+    NUSHT_REQUIRE(p->rp->spin() == 0 || map1, kErrArg, "spin > 0 needs two input maps");
+    p->rp->adjoint(map0, map1, reinterpret_cast<double2*>(alm), ncomp, S(s));
+  });
+}
+
+int nusht_ringplan_dlm2pointing(nusht_ringplan_t p, const double* dlm_scaled, double* ptg_theta, double* ptg_phi,
+                                double* gamma, nusht_stream_t s) {
+  return guard([&] {
+    NUSHT_REQUIRE(p, kErrArg, "null ring plan");
+    p->rp->dlm2pointing(reinterpret_cast<const double2*>(dlm_scaled), ptg_theta, ptg_phi, gamma, S(s));
+  });
+}
+
+// ------------------------------------------------------------------ nuSHT plan
+int nusht_plan_create(const nusht_plan_desc* d, nusht_stream_t s, nusht_plan_t* out) {
+  return guard([&] {
+    NUSHT_REQUIRE(d && out, kErrArg, "null argument");
+    NUSHT_REQUIRE(d->mmax == d->lmax, kErrArg, "mmax must equal lmax");
+    NUSHT_REQUIRE(d->spin >= 0 && d->spin <= 2, kErrArg, "spin must be 0, 1 or 2");
+    NUSHT_REQUIRE(d->ncomp == 1 || (d->ncomp == 2 && d->spin > 0), kErrArg, "ncomp must be 1, or 2 for spin > 0");
+    auto p = std::make_unique<nusht_plan_s>();
+    p->lmax = d->lmax;
+    p->spin = d->spin;
+    p->ncomp = d->ncomp;
+    p->npts_max = std::max<long>(d->npts_max, 1);
+    p->np = NufftParams::choose(d->lmax, d->spin, d->epsilon, d->sigma);
+    std::vector<double> th(p->np.ntheta);
+    for (int j = 0; j < p->np.ntheta; ++j) th[j] = M_PI * j / (p->np.ntheta - 1);
+    p->tab.build(d->lmax, d->spin, th, S(s));
+    p->nu = std::make_unique<NufftPlan>(p->np, p->npts_max, S(s));
+    p->nmrows = d->spin ? 2 * d->lmax + 1 : d->lmax + 1;
+    p->coef.alloc(legendre_coef_bytes(p->tab, nusht_plan_s::kMaxMaps));
+    p->G.alloc(sizeof(double2) * (size_t)p->nmrows * p->np.ntheta * nusht_plan_s::kMaxMaps);
+    p->ptg.alloc(sizeof(double) * 3 * (size_t)p->npts_max);
+    NUSHT_HIP(hipStreamSynchronize(S(s)));
+    *out = p.release();
+  });
+}
+
+int nusht_plan_destroy(nusht_plan_t p) {
+  return guard([&] { delete p; });
+}
+
+int nusht_plan_get_info(nusht_plan_t p, nusht_plan_info* info) {
+  return guard([&] {
+    check_plan(p);
+    NUSHT_REQUIRE(info, kErrArg, "null info");
+    info->ntheta = p->np.ntheta;
+    info->ntheta_dfs = p->np.ntd;
+    info->nfine = p->np.nf;
+    info->nrow_fine = p->np.nrow;
+    info->kernel_width = p->np.w;
+    info->single_precision = p->np.single ? 1 : 0;
+    info->sigma = p->np.sigma;
+    info->beta = p->np.beta;
+    info->workspace_bytes = (int64_t)(p->tab.bytes() + p->nu->bytes() + p->coef.bytes() + p->G.bytes() + p->ptg.bytes());
+  });
+}
+
+int nusht_plan_set_pts(nusht_plan_t p, const double* theta, const double* phi, int64_t npts, nusht_stream_t s) {
+  return guard([&] {
+    check_plan(p);
+    p->nu->set_pts(theta, phi, npts, S(s));
+  });
+}
+
+static void synth_batch(nusht_plan_t p, int nmaps, const double* alm, long alm_stride, double* out0, double* out1,
+                        long out_stride, const double* gamma, hipStream_t st) {
+  const long gmap = (long)p->nmrows * p->np.ntheta;
+  legendre_synth(p->tab, nmaps, p->ncomp, reinterpret_cast<const double2*>(alm), alm_stride / 2, nullptr,
+                 p->coef.as<double2>(), p->G.as<double2>(), p->np.ntheta, 1, gmap, st);
+  for (int b = 0; b < nmaps; ++b)
+    p->nu->type2(p->G.as<double2>() + b * gmap, p->np.ntheta, gamma, out0 + b * out_stride,
+                 out1 ? out1 + b * out_stride : nullptr, st);
+}
+
+int nusht_nusht2d2(nusht_plan_t p, const double* alm, double* out0, double* out1, const double* gamma,
+                   nusht_stream_t s) {
+  return guard([&] {
+    check_plan(p);
+    NUSHT_REQUIRE(alm && out0, kErrArg, "null array");
+    NUSHT_REQUIRE(p->spin == 0 || out1, kErrArg, "spin > 0 needs two output arrays");
+    synth_batch(p, 1, alm, 0, out0, out1, 0, gamma, S(s));
+  });
+}
+
+int nusht_nusht2d2_batch(nusht_plan_t p, int nmaps, const double* alm, int64_t alm_stride, double* out0,
+                         double* out1, int64_t out_stride, const double* gamma, nusht_stream_t s) {
+  return guard([&] {
+    check_plan(p);
+    NUSHT_REQUIRE(nmaps >= 1 && nmaps <= nusht_plan_s::kMaxMaps, kErrArg, "nmaps must be in [1, 8]");
+    NUSHT_REQUIRE(alm && out0, kErrArg, "null array");
+    NUSHT_REQUIRE(p->spin == 0 || out1, kErrArg, "spin > 0 needs two output arrays");
+    synth_batch(p, nmaps, alm, alm_stride, out0, out1, out_stride, gamma, S(s));
+  });
+}
+
+int nusht_nusht2d1(nusht_plan_t p, const double* in0, const double* in1, const double* gamma, double* alm,
+                   nusht_stream_t s) {
+  return guard([&] {
+    check_plan(p);
+    NUSHT_REQUIRE(in0 && alm, kErrArg, "null array");
+    NUSHT_REQUIRE(p->spin == 0 || in1, kErrArg, "spin > 0 needs two input arrays");
+    p->nu->type1(in0, in1, gamma, p->G.as<double2>(), p->np.ntheta, S(s));
+    legendre_adjoint(p->tab, p->ncomp, p->G.as<double2>(), p->np.ntheta, 1, reinterpret_cast<double2*>(alm), S(s));
+  });
+}
+
+int nusht_gclm2lenmap(nusht_plan_t p, nusht_ringplan_t ptg, const double* gclm, const double* dlm_scaled,
+                      double* lenmap0, double* lenmap1, int polrot, nusht_stream_t s) {
+  return guard([&] {
+    check_plan(p);
+    NUSHT_REQUIRE(ptg && ptg->rp->spin() == 1, kErrArg, "need a spin-1 ring plan for the pointing");
+    NUSHT_REQUIRE(ptg->rp->lmax() == p->lmax, kErrArg, "ring plan lmax differs from the nuSHT plan");
+    const long npix = ptg->rp->npix();
+    NUSHT_REQUIRE(npix <= p->npts_max, kErrArg, "pointing grid larger than the plan's npts_max");
+    double* th = p->ptg.as<double>();
+    double* ph = th + p->npts_max;
+    double* ga = ph + p->npts_max;
+    const bool rot = polrot && p->spin > 0;
+    ptg->rp->dlm2pointing(reinterpret_cast<const double2*>(dlm_scaled), th, ph, rot ? ga : nullptr, S(s));
+    p->nu->set_pts(th, ph, npix, S(s));
+    synth_batch(p, 1, gclm, 0, lenmap0, lenmap1, 0, rot ? ga : nullptr, S(s));
+  });
+}
+
+int nusht_lenmap2gclm(nusht_plan_t p, nusht_ringplan_t ptg, const double* lenmap0, const double* lenmap1,
+                      const double* dlm_scaled, double* gclm, int polrot, nusht_stream_t s) {
+  return guard([&] {
+    check_plan(p);
+    NUSHT_REQUIRE(ptg && ptg->rp->spin() == 1, kErrArg, "need a spin-1 ring plan for the pointing");
+    NUSHT_REQUIRE(ptg->rp->lmax() == p->lmax, kErrArg, "ring plan lmax differs from the nuSHT plan");
+    const long npix = ptg->rp->npix();
+    NUSHT_REQUIRE(npix <= p->npts_max, kErrArg, "pointing grid larger than the plan's npts_max");
+    double* th = p->ptg.as<double>();
+    double* ph = th + p->npts_max;
+    double* ga = ph + p->npts_max;
+    const bool rot = polrot && p->spin > 0;
+    ptg->rp->dlm2pointing(reinterpret_cast<const double2*>(dlm_scaled), th, ph, rot ? ga : nullptr, S(s));
+    p->nu->set_pts(th, ph, npix, S(s));
+    p->nu->type1(lenmap0, lenmap1, rot ? ga : nullptr, p->G.as<double2>(), p->np.ntheta, S(s));
+    legendre_adjoint(p->tab, p->ncomp, p->G.as<double2>(), p->np.ntheta, 1, reinterpret_cast<double2*>(gclm), S(s));
+  });
+}
+
+}  // extern "C"

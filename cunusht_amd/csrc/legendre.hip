@@ -1,0 +1,637 @@
+// Spin-weighted Legendre ring transforms on gfx950 (FP64).  See legendre.h for the conventions.
+//
+// Replaces the reference's SHTns synth / analys / synth_grad calls
+// (cunusht/sht/GPU_sht_transformer.py:56-96) with per-m three-term recursions:
+//   synthesis : one wave per (m, block of 64 ring pairs); coefficients are wave-uniform
+//               and arrive through the scalar cache; the l-loop runs in two phases, first
+//               with underflow rescaling while any lane is still below 2^-300, then plain.
+//   adjoint   : one workgroup per m holding all ring pairs (4 per lane); per-l sums over
+//               lanes are formed with a butterfly transpose-reduce in registers, summed over
+//               waves with LDS atomics, then written once.
+#include "legendre.h"
+
+#include <algorithm>
+#include <cmath>
+
+namespace nusht {
+namespace {
+
+__host__ __device__ inline int l0_of(int m, int s) { return m > s ? m : s; }
+
+constexpr double kThresh = 0x1p300;   // rescale when a scaled value exceeds this
+constexpr double kSmall = 0x1p-600;   // one scale step
+constexpr double kSeedLo = 0x1p-300;
+
+// ------------------------------------------------------------------ tables
+__global__ void k_rec_tables(int lmax, int s, const long* __restrict__ mofs, double2* __restrict__ rec,
+                             double* __restrict__ cnorm) {
+  int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m > lmax) return;
+  int l0 = l0_of(m, s);
+  if (l0 > lmax) return;
+  long base = mofs[m];
+  double cprev = 1.0, ccur = 1.0;
+  cnorm[base] = 1.0;
+  for (int l = l0; l < lmax; ++l) {
+    double alpha, b, gamma;
+    if (l == 0) {
+      alpha = sqrt(3.0); b = 0.0; gamma = 0.0;
+    } else {
+      double d1 = sqrt(((l + 1.0) * (l + 1) - (double)m * m) * ((l + 1.0) * (l + 1) - (double)s * s));
+      double d0 = sqrt(((double)l * l - (double)m * m) * ((double)l * l - (double)s * s));
+      double f = sqrt((2.0 * l + 3) * (2.0 * l + 1));
+      alpha = f * (l + 1) / d1;
+      b = f * (double)m * s / (l * d1);
+      gamma = sqrt((2.0 * l + 3) / (2.0 * l - 1)) * (l + 1) * d0 / (l * d1);
+    }
+    double cnext = (l == l0) ? 1.0 : gamma * cprev;
+    rec[base + (l - l0)] = double2{alpha * ccur / cnext, b * ccur / cnext};
+    cnorm[base + (l - l0) + 1] = cnext;
+    cprev = ccur;
+    ccur = cnext;
+  }
+}
+
+__device__ inline void renorm(double& v, double& k) {
+  while (v != 0.0 && fabs(v) < kSeedLo) { v *= 0x1p600; k -= 1.0; }
+}
+
+// Starting values at l0 = max(m, s): u = N_l0 d^l0_{m,-s}, v = N_l0 d^l0_{m,+s}, with scale exponents.
+__global__ __launch_bounds__(64) void k_seeds(int lmax, int s, int nslot, const double* __restrict__ slot_theta,
+                        double4* __restrict__ seeds) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nslot) return;
+  const double th = slot_theta[i];
+  const double sh = sin(0.5 * th), ch = cos(0.5 * th);
+  const double ns = sqrt((2.0 * s + 1) / (4.0 * M_PI));
+  double u = 0, v = 0, ku = 0, kv = 0;
+  // m < s: closed form at l0 = s
+  for (int m = 0; m < s && m <= lmax; ++m) {
+    double lg = 0.5 * (lgamma(2.0 * s + 1) - lgamma(s + m + 1.0) - lgamma(s - m + 1.0));
+    double bin = exp(lg);
+    double uu = ns * bin * pow(ch, (double)(s - m)) * pow(sh, (double)(s + m));
+    if ((m + s) & 1) uu = -uu;
+    double vv = ns * bin * pow(ch, (double)(s + m)) * pow(sh, (double)(s - m));
+    seeds[(long)m * nslot + i] = double4{uu, vv, 0.0, 0.0};
+  }
+  if (s > lmax) return;
+  // m = s: u = N_s sin^{2s}(th/2), v = N_s cos^{2s}(th/2); then the product recursion in m.
+  u = ns; v = ns;
+  for (int j = 0; j < 2 * s; ++j) { u *= sh; v *= ch; }
+  renorm(u, ku);
+  renorm(v, kv);
+  seeds[(long)s * nslot + i] = double4{u, v, ku, kv};
+  for (int m = s + 1; m <= lmax; ++m) {
+    double f = -sqrt(2.0 * m * (2.0 * m + 1) / ((double)m * m - (double)s * s)) * sh * ch;
+    u *= f;
+    v *= f;
+    renorm(u, ku);
+    renorm(v, kv);
+    seeds[(long)m * nslot + i] = double4{u, v, ku, kv};
+  }
+}
+
+// ------------------------------------------------------------------ synthesis prep
+// coef layout per map b, (m, l):  spin 0: coef[(mofs[m]+l-l0)*NB + b] = a_lm c_l fl_l
+//                                 spin s: coef[((mofs[m]+l-l0)*NB + b)*2 + {0,1}] = (W', Wt')
+//   W' = -(-1)^s c_l (E + iB),  Wt' = -c_l (conj E + i conj B).
+__global__ void k_prep(int lmax, int s, int nmaps, int nbpad, int ncomp, const double2* __restrict__ alm, long stride_map,
+                       const double* __restrict__ fl, const long* __restrict__ mofs,
+                       const double* __restrict__ cnorm, double2* __restrict__ coef) {
+  const int m = blockIdx.y;
+  const int l0 = l0_of(m, s);
+  if (l0 > lmax) return;
+  const long nalm = (long)(lmax + 1) * (lmax + 2) / 2;
+  const long mbase = (long)m * (2 * lmax + 1 - m) / 2;
+  const double sg = (s & 1) ? 1.0 : -1.0;  // -(-1)^s
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j <= lmax - l0; j += gridDim.x * blockDim.x) {
+    const int l = l0 + j;
+    const long q = mofs[m] + j;
+    double c = cnorm[q];
+    if (fl) c *= fl[l];
+    for (int b = 0; b < nbpad; ++b) {
+      double2 e{0.0, 0.0}, bb{0.0, 0.0};
+      if (b < nmaps) {
+        const double2* ab = alm + b * stride_map;
+        e = ab[mbase + l];
+        if (s != 0 && ncomp > 1) bb = ab[nalm + mbase + l];
+      }
+      if (s == 0) {
+        coef[q * nbpad + b] = double2{e.x * c, e.y * c};
+      } else {
+        double2 w{e.x - bb.y, e.y + bb.x};
+        double2 wt{e.x + bb.y, -e.y + bb.x};
+        coef[(q * nbpad + b) * 2] = double2{sg * c * w.x, sg * c * w.y};
+        coef[(q * nbpad + b) * 2 + 1] = double2{-c * wt.x, -c * wt.y};
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ synthesis kernel
+template <int NB>
+struct SynAcc0 {
+  double2 a[2][NB];
+};
+
+__device__ __forceinline__ void fma2(double2& acc, double2 c, double w) {
+  acc.x = fma(c.x, w, acc.x);
+  acc.y = fma(c.y, w, acc.y);
+}
+
+struct SynthArgs {
+  int lmax, s, nslot, nsb, nmaps;
+  const double* slot_theta;
+  const int* slot_north;
+  const int* slot_south;
+  const long* mofs;
+  const double2* rec;
+  const double4* seeds;
+  const double2* coef;
+  double2* G;
+  long sm, sr, smap;
+};
+
+template <bool SP, int NB>
+__global__ __launch_bounds__(64) void k_synth(SynthArgs a) {
+  const int blk = blockIdx.x;
+  const int m = blk / a.nsb;
+  const int sb = blk - m * a.nsb;
+  const int slot = sb * 64 + threadIdx.x;
+  const bool active = slot < a.nslot;
+  const int L = a.lmax;
+  const int l0 = l0_of(m, a.s);
+  if (l0 > L) return;
+
+  double x = 0.0, mu_u = 0.0, mu_v = 0.0, ku = 0.0, kv = 0.0;
+  if (active) {
+    x = cos(a.slot_theta[slot]);
+    double4 sd = a.seeds[(long)m * a.nslot + slot];
+    mu_u = sd.x; mu_v = sd.y; ku = sd.z; kv = sd.w;
+  }
+  if (!SP) { mu_v = 0.0; kv = 0.0; }
+  double pu = 0.0, pv = 0.0;
+
+  const long base = a.mofs[m];
+  const double2* __restrict__ rec = a.rec + base;
+  constexpr int K = SP ? 2 * NB : NB;
+  const double2* __restrict__ cf = a.coef + base * K;
+
+  // accumulators: spin 0 -> acc[parity][b]; spin s -> N (no sign) and S (parity split) for +m and -m
+  double2 A0[NB], A1[NB];          // spin 0: parity-split sums; spin s: +m south parity sums
+  double2 PN[NB], MN[NB], MS0[NB], MS1[NB];
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    A0[b] = A1[b] = double2{0.0, 0.0};
+    PN[b] = MN[b] = MS0[b] = MS1[b] = double2{0.0, 0.0};
+  }
+
+  int l = l0;
+  int rel = 0;  // (l - l0) parity toggle
+
+  auto contrib = [&](int j, double wu, double wv, bool odd) {
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      if (!SP) {
+        double2 c = cf[(long)j * NB + b];
+        if (odd) fma2(A1[b], c, wu); else fma2(A0[b], c, wu);
+      } else {
+        double2 w = cf[((long)j * NB + b) * 2];
+        double2 wt = cf[((long)j * NB + b) * 2 + 1];
+        fma2(PN[b], w, wu);
+        fma2(MN[b], wt, wv);
+        if (odd) { fma2(A1[b], w, wv); fma2(MS1[b], wt, wu); }
+        else { fma2(A0[b], w, wv); fma2(MS0[b], wt, wu); }
+      }
+    }
+  };
+
+  // phase 1: some lane still underflowed -> masked contributions and rescaling
+  while (l <= L && __any((ku < 0.0) || (kv < 0.0))) {
+    const int j = l - l0;
+    contrib(j, ku == 0.0 ? mu_u : 0.0, kv == 0.0 ? mu_v : 0.0, rel);
+    if (l < L) {
+      double2 r = rec[j];
+      double t = r.x * x;
+      double nu = fma(t + r.y, mu_u, -pu);
+      pu = mu_u; mu_u = nu;
+      if (ku < 0.0 && fabs(mu_u) > kThresh) { mu_u *= kSmall; pu *= kSmall; ku += 1.0; }
+      if (SP) {
+        double nv = fma(t - r.y, mu_v, -pv);
+        pv = mu_v; mu_v = nv;
+        if (kv < 0.0 && fabs(mu_v) > kThresh) { mu_v *= kSmall; pv *= kSmall; kv += 1.0; }
+      }
+    }
+    ++l;
+    rel ^= 1;
+  }
+  // align to an even (l - l0) so that the unrolled pair is (even, odd)
+  if (rel && l <= L) {
+    const int j = l - l0;
+    contrib(j, mu_u, mu_v, true);
+    if (l < L) {
+      double2 r = rec[j];
+      double t = r.x * x;
+      double nu = fma(t + r.y, mu_u, -pu);
+      pu = mu_u; mu_u = nu;
+      if (SP) { double nv = fma(t - r.y, mu_v, -pv); pv = mu_v; mu_v = nv; }
+    }
+    ++l;
+    rel = 0;
+  }
+  // phase 2: plain recursion, unrolled by two
+  for (; l + 1 <= L; l += 2) {
+    const int j = l - l0;
+    double2 r0 = rec[j];
+    double2 r1 = rec[j + 1];
+    contrib(j, mu_u, mu_v, false);
+    double t = r0.x * x;
+    double nu = fma(t + r0.y, mu_u, -pu);
+    pu = mu_u; mu_u = nu;
+    if (SP) { double nv = fma(t - r0.y, mu_v, -pv); pv = mu_v; mu_v = nv; }
+    contrib(j + 1, mu_u, mu_v, true);
+    if (l + 1 < L) {
+      t = r1.x * x;
+      nu = fma(t + r1.y, mu_u, -pu);
+      pu = mu_u; mu_u = nu;
+      if (SP) { double nv = fma(t - r1.y, mu_v, -pv); pv = mu_v; mu_v = nv; }
+    }
+  }
+  if (l <= L) contrib(l - l0, mu_u, mu_v, false);
+
+  if (!active) return;
+  const int rn = a.slot_north[slot];
+  const int rs = a.slot_south[slot];
+  const double sg0 = ((l0 + m) & 1) ? -1.0 : 1.0;  // (-1)^{l+m} for even (l - l0)
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+    if (b >= a.nmaps) break;
+    double2* Gb = a.G + b * a.smap;
+    if (!SP) {
+      double2 gn = cadd(A0[b], A1[b]);
+      double2 gs = double2{sg0 * (A0[b].x - A1[b].x), sg0 * (A0[b].y - A1[b].y)};
+      Gb[(long)m * a.sm + (long)rn * a.sr] = gn;
+      if (rs >= 0) Gb[(long)m * a.sm + (long)rs * a.sr] = gs;
+    } else {
+      double2 pS = double2{sg0 * (A0[b].x - A1[b].x), sg0 * (A0[b].y - A1[b].y)};
+      double2 mS = double2{sg0 * (MS0[b].x - MS1[b].x), sg0 * (MS0[b].y - MS1[b].y)};
+      const long rowp = (long)(L + m) * a.sm, rowm = (long)(L - m) * a.sm;
+      Gb[rowp + (long)rn * a.sr] = PN[b];
+      if (rs >= 0) Gb[rowp + (long)rs * a.sr] = pS;
+      if (m > 0) {
+        Gb[rowm + (long)rn * a.sr] = MN[b];
+        if (rs >= 0) Gb[rowm + (long)rs * a.sr] = mS;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ adjoint kernel
+template <bool SP> struct AdjCfg {
+  static constexpr int R = SP ? 3 : 4;          // ring pairs per lane per pass
+  static constexpr int MaxThreads = 768;
+};
+
+// Sum of K complex values per lane over the 64 lanes of a wave (butterfly transpose-reduce).
+// Each step exchanges half of the remaining values with lane ^ off and keeps the other half,
+// so after log2(K) steps every lane holds one fully lane-summed element; the remaining steps
+// are plain all-reduce butterflies.  Lane ln ends up with element reduce_index<K>(ln).
+template <int K, int k, int off>
+__device__ __forceinline__ void tr_step(double2 (&v)[K], int ln) {
+  if constexpr (off > 0) {
+    if constexpr (k > 1) {
+      constexpr int half = k / 2;
+      const bool h = (ln & off) != 0;
+#pragma unroll
+      for (int i = 0; i < half; ++i) {
+        double2 keep = h ? v[i + half] : v[i];
+        double2 send = h ? v[i] : v[i + half];
+        double rx = __shfl_xor(send.x, off, 64);
+        double ry = __shfl_xor(send.y, off, 64);
+        v[i] = double2{keep.x + rx, keep.y + ry};
+      }
+      tr_step<K, half, off / 2>(v, ln);
+    } else {
+      v[0].x += __shfl_xor(v[0].x, off, 64);
+      v[0].y += __shfl_xor(v[0].y, off, 64);
+      tr_step<K, 1, off / 2>(v, ln);
+    }
+  }
+}
+
+template <int K>
+__device__ __forceinline__ double2 transpose_reduce(double2 (&v)[K]) {
+  tr_step<K, K, 32>(v, (int)(threadIdx.x & 63));
+  return v[0];
+}
+
+template <int K>
+__device__ __forceinline__ int reduce_index() {
+  const int ln = threadIdx.x & 63;
+  int idx = 0, k = K, off = 32;
+#pragma unroll
+  for (int step = 0; step < 6; ++step) {
+    if (k > 1) {
+      int half = k / 2;
+      if (ln & off) idx += half;
+      k = half;
+    }
+    off >>= 1;
+  }
+  return idx;
+}
+
+struct AdjArgs {
+  int lmax, s, nslot, ncomp;
+  const double* slot_theta;
+  const int* slot_north;
+  const int* slot_south;
+  const long* mofs;
+  const double2* rec;
+  const double* cnorm;
+  const double4* seeds;
+  const double2* G;
+  long sm, sr;
+  double2* alm;
+};
+
+template <bool SP>
+__global__ __launch_bounds__(AdjCfg<SP>::MaxThreads) void k_adjoint(AdjArgs a) {
+  constexpr int kAdjR = AdjCfg<SP>::R;
+  extern __shared__ double2 lacc[];
+  const int m = blockIdx.x;
+  const int L = a.lmax;
+  const int l0 = l0_of(m, a.s);
+  const long nalm = (long)(L + 1) * (L + 2) / 2;
+  const long mbase = (long)m * (2 * L + 1 - m) / 2;
+  if (l0 > L) {
+    for (int l = m + threadIdx.x; l <= L; l += blockDim.x) {
+      for (int c = 0; c < a.ncomp; ++c) a.alm[c * nalm + mbase + l] = double2{0.0, 0.0};
+    }
+    return;
+  }
+  const int nl = L - l0 + 1;
+  constexpr int NA = SP ? 2 : 1;
+  for (int i = threadIdx.x; i < nl * NA; i += blockDim.x) lacc[i] = double2{0.0, 0.0};
+
+  __syncthreads();
+  const long base = a.mofs[m];
+  const double2* __restrict__ rec = a.rec + base;
+  const double sgA = ((l0 + m) & 1) ? -1.0 : 1.0;  // (-1)^{l+m} at l = l0
+  constexpr int K = SP ? 4 : 8;
+  const int ridx = reduce_index<K>();
+  const bool writer = ((threadIdx.x & 63) & (64 / K - 1)) == 0;
+  const int nthr = blockDim.x;
+  const int cap = nthr * kAdjR;
+
+  for (int pass0 = 0; pass0 < a.nslot; pass0 += cap) {
+  // per-lane ring pairs
+  double x[kAdjR], mu[kAdjR], pm[kAdjR], ku[kAdjR];
+  double nv_[kAdjR], pv[kAdjR], kv[kAdjR];
+  double2 P0[kAdjR], P1[kAdjR], Q0[kAdjR], Q1[kAdjR];  // spin 0: E/O in P0/P1; spin s: PN, PS, QN, QS
+#pragma unroll
+  for (int r = 0; r < kAdjR; ++r) {
+    const int slot = pass0 + r * nthr + threadIdx.x;
+    x[r] = mu[r] = pm[r] = ku[r] = nv_[r] = pv[r] = kv[r] = 0.0;
+    P0[r] = P1[r] = Q0[r] = Q1[r] = double2{0.0, 0.0};
+    if (slot < a.nslot) {
+      x[r] = cos(a.slot_theta[slot]);
+      double4 sd = a.seeds[(long)m * a.nslot + slot];
+      mu[r] = sd.x; ku[r] = sd.z;
+      if (SP) { nv_[r] = sd.y; kv[r] = sd.w; }
+      const int rn = a.slot_north[slot];
+      int rs = a.slot_south[slot];
+      if (rs == rn) rs = -1;
+      if (!SP) {
+        double2 gn = a.G[(long)m * a.sm + (long)rn * a.sr];
+        double2 gs = rs >= 0 ? a.G[(long)m * a.sm + (long)rs * a.sr] : double2{0.0, 0.0};
+        P0[r] = cadd(gn, gs);
+        P1[r] = csub(gn, gs);
+      } else {
+        const long rowp = (long)(L + m) * a.sm, rowm = (long)(L - m) * a.sm;
+        P0[r] = a.G[rowp + (long)rn * a.sr];
+        P1[r] = rs >= 0 ? a.G[rowp + (long)rs * a.sr] : double2{0.0, 0.0};
+        Q0[r] = cconj(a.G[rowm + (long)rn * a.sr]);
+        Q1[r] = rs >= 0 ? cconj(a.G[rowm + (long)rs * a.sr]) : double2{0.0, 0.0};
+      }
+    }
+  }
+  for (int lc = l0; lc <= L; lc += K) {
+    double2 X[K], Y[K];
+#pragma unroll
+    for (int i = 0; i < K; ++i) X[i] = double2{0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < K; ++i) Y[i] = double2{0.0, 0.0};
+    bool slow = false;
+#pragma unroll
+    for (int r = 0; r < kAdjR; ++r) slow |= (ku[r] < 0.0) || (kv[r] < 0.0);
+    slow = __any(slow);
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      const int l = lc + i;
+      if (l <= L) {
+        const double sg = (i & 1) ? -sgA * (((lc - l0) & 1) ? -1.0 : 1.0) : sgA * (((lc - l0) & 1) ? -1.0 : 1.0);
+        const double2 rc = (l < L) ? rec[l - l0] : double2{0.0, 0.0};
+#pragma unroll
+        for (int r = 0; r < kAdjR; ++r) {
+          double wu = mu[r], wv = nv_[r];
+          if (slow) { wu = ku[r] == 0.0 ? wu : 0.0; wv = kv[r] == 0.0 ? wv : 0.0; }
+          if (!SP) {
+            double2 p = sg > 0 ? P0[r] : P1[r];
+            fma2(X[i], p, wu);
+          } else {
+            fma2(X[i], P0[r], wu);
+            fma2(X[i], double2{sg * P1[r].x, sg * P1[r].y}, wv);
+            fma2(Y[i], Q0[r], wv);
+            fma2(Y[i], double2{sg * Q1[r].x, sg * Q1[r].y}, wu);
+          }
+          if (l < L) {
+            double t = rc.x * x[r];
+            double nu = fma(t + rc.y, mu[r], -pm[r]);
+            pm[r] = mu[r]; mu[r] = nu;
+            if (SP) { double nn = fma(t - rc.y, nv_[r], -pv[r]); pv[r] = nv_[r]; nv_[r] = nn; }
+            if (slow) {
+              if (ku[r] < 0.0 && fabs(mu[r]) > kThresh) { mu[r] *= kSmall; pm[r] *= kSmall; ku[r] += 1.0; }
+              if (SP && kv[r] < 0.0 && fabs(nv_[r]) > kThresh) { nv_[r] *= kSmall; pv[r] *= kSmall; kv[r] += 1.0; }
+            }
+          }
+        }
+      }
+    }
+    double2 xs = transpose_reduce<K>(X);
+    if (writer) {
+      const int l = lc + ridx;
+      if (l <= L) {
+        atomicAdd(&lacc[(l - l0) * NA].x, xs.x);
+        atomicAdd(&lacc[(l - l0) * NA].y, xs.y);
+      }
+    }
+    if constexpr (SP) {
+      double2 ys = transpose_reduce<K>(Y);
+      if (writer) {
+        const int l = lc + ridx;
+        if (l <= L) {
+          atomicAdd(&lacc[(l - l0) * NA + 1].x, ys.x);
+          atomicAdd(&lacc[(l - l0) * NA + 1].y, ys.y);
+        }
+      }
+    }
+  }
+  }  // pass
+  __syncthreads();
+
+  // write out (also zeros for m <= l < l0)
+  const double sgs = (a.s & 1) ? -1.0 : 1.0;
+  for (int l = m + threadIdx.x; l <= L; l += blockDim.x) {
+    double2 e{0.0, 0.0}, bb{0.0, 0.0};
+    if (l >= l0) {
+      const double c = a.cnorm[base + (l - l0)];
+      if (!SP) {
+        e = lacc[l - l0];
+        e = double2{c * e.x, m == 0 ? 0.0 : c * e.y};
+      } else {
+        double2 X = lacc[(l - l0) * 2], Y = lacc[(l - l0) * 2 + 1];
+        X = double2{sgs * c * X.x, sgs * c * X.y};
+        Y = double2{c * Y.x, c * Y.y};
+        e = double2{-0.5 * (X.x + Y.x), -0.5 * (X.y + Y.y)};
+        bb = double2{-0.5 * (X.y - Y.y), 0.5 * (X.x - Y.x)};
+      }
+    }
+    a.alm[mbase + l] = e;
+    if (SP && a.ncomp > 1) a.alm[nalm + mbase + l] = bb;
+  }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ host
+void LegendreTables::build(int L, int s, const std::vector<double>& theta, hipStream_t st) {
+  NUSHT_REQUIRE(L >= 0 && s >= 0 && s <= 3, kErrArg, "legendre: bad lmax/spin");
+  lmax = L;
+  spin = s;
+  nring = (int)theta.size();
+  // pair rings symmetric about the equator
+  std::vector<int> order(nring);
+  for (int i = 0; i < nring; ++i) order[i] = i;
+  std::sort(order.begin(), order.end(), [&](int a, int b) { return theta[a] < theta[b]; });
+  std::vector<char> used(nring, 0);
+  std::vector<double> sth;
+  std::vector<int> sn, ss;
+  int lo = 0, hi = nring - 1;
+  while (lo <= hi) {
+    int a = order[lo], b = order[hi];
+    if (lo == hi) {
+      sth.push_back(theta[a]); sn.push_back(a); ss.push_back(a);
+      break;
+    }
+    if (std::fabs(theta[a] + theta[b] - M_PI) < 1e-13) {
+      sth.push_back(theta[a]); sn.push_back(a); ss.push_back(b);
+      ++lo; --hi;
+    } else if (theta[a] + theta[b] < M_PI) {
+      sth.push_back(theta[a]); sn.push_back(a); ss.push_back(-1);
+      ++lo;
+    } else {
+      sth.push_back(theta[b]); sn.push_back(b); ss.push_back(-1);
+      --hi;
+    }
+  }
+  // an equator ring paired with itself: keep it as north only
+  for (size_t i = 0; i < sn.size(); ++i)
+    if (ss[i] == sn[i]) ss[i] = -1;
+  nslot = (int)sth.size();
+  upload(slot_theta, sth);
+  upload(slot_north, sn);
+  upload(slot_south, ss);
+
+  std::vector<long> mo(L + 2, 0);
+  for (int m = 0; m <= L; ++m) {
+    int l0 = l0_of(m, s);
+    mo[m + 1] = mo[m] + (l0 <= L ? (L - l0 + 1) : 0);
+  }
+  nlm = mo[L + 1];
+  upload(mofs, mo);
+  rec.alloc(sizeof(double2) * (size_t)std::max<long>(nlm, 1));
+  cnorm.alloc(sizeof(double) * (size_t)std::max<long>(nlm, 1));
+  seeds.alloc(sizeof(double4) * (size_t)(L + 1) * nslot);
+  hipLaunchKernelGGL(k_rec_tables, dim3(ceil_div(L + 1, 128)), dim3(128), 0, st, L, s, mofs.as<long>(),
+                     rec.as<double2>(), cnorm.as<double>());
+  NUSHT_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_seeds, dim3(ceil_div(nslot, 64)), dim3(64), 0, st, L, s, nslot, slot_theta.as<double>(),
+                     seeds.as<double4>());
+  NUSHT_LAUNCH_CHECK();
+}
+
+static int padded_maps(int nmaps) { return nmaps <= 1 ? 1 : nmaps <= 2 ? 2 : nmaps <= 4 ? 4 : 8; }
+
+size_t legendre_coef_bytes(const LegendreTables& t, int nmaps) {
+  return sizeof(double2) * (size_t)std::max<long>(t.nlm, 1) * padded_maps(nmaps) * (t.spin ? 2 : 1);
+}
+
+template <bool SP, int NB>
+static void launch_synth(const SynthArgs& a, int nblocks, hipStream_t st) {
+  hipLaunchKernelGGL((k_synth<SP, NB>), dim3(nblocks), dim3(64), 0, st, a);
+  NUSHT_LAUNCH_CHECK();
+}
+
+void legendre_synth(const LegendreTables& t, int nmaps, int ncomp, const double2* alm, long alm_stride_map,
+                    const double* fl, double2* coef_ws, double2* G, long sm, long sr, long smap, hipStream_t st) {
+  NUSHT_REQUIRE(nmaps >= 1 && nmaps <= 8, kErrArg, "legendre_synth: nmaps must be in [1, 8]");
+  const int L = t.lmax;
+  const int nbpad = padded_maps(nmaps);
+  {
+    dim3 grid(ceil_div(L + 1, 256), L + 1);
+    hipLaunchKernelGGL(k_prep, grid, dim3(256), 0, st, L, t.spin, nmaps, nbpad, ncomp, alm, alm_stride_map, fl,
+                       t.mofs.as<long>(), t.cnorm.as<double>(), coef_ws);
+    NUSHT_LAUNCH_CHECK();
+  }
+  SynthArgs a;
+  a.lmax = L; a.s = t.spin; a.nslot = t.nslot; a.nsb = ceil_div(t.nslot, 64); a.nmaps = nmaps;
+  a.slot_theta = t.slot_theta.as<double>();
+  a.slot_north = t.slot_north.as<int>();
+  a.slot_south = t.slot_south.as<int>();
+  a.mofs = t.mofs.as<long>();
+  a.rec = t.rec.as<double2>();
+  a.seeds = t.seeds.as<double4>();
+  a.coef = coef_ws;
+  a.G = G; a.sm = sm; a.sr = sr; a.smap = smap;
+  const int nblocks = a.nsb * (L + 1);
+  const bool sp = t.spin > 0;
+  switch (nbpad) {  // the kernel's NB must equal the coefficient stride written by k_prep
+    case 1: sp ? launch_synth<true, 1>(a, nblocks, st) : launch_synth<false, 1>(a, nblocks, st); break;
+    case 2: sp ? launch_synth<true, 2>(a, nblocks, st) : launch_synth<false, 2>(a, nblocks, st); break;
+    case 4: sp ? launch_synth<true, 4>(a, nblocks, st) : launch_synth<false, 4>(a, nblocks, st); break;
+    default: sp ? launch_synth<true, 8>(a, nblocks, st) : launch_synth<false, 8>(a, nblocks, st); break;
+  }
+}
+
+void legendre_adjoint(const LegendreTables& t, int ncomp, const double2* G, long sm, long sr, double2* alm,
+                      hipStream_t st) {
+  const int L = t.lmax;
+  AdjArgs a;
+  a.lmax = L; a.s = t.spin; a.nslot = t.nslot; a.ncomp = ncomp;
+  a.slot_theta = t.slot_theta.as<double>();
+  a.slot_north = t.slot_north.as<int>();
+  a.slot_south = t.slot_south.as<int>();
+  a.mofs = t.mofs.as<long>();
+  a.rec = t.rec.as<double2>();
+  a.cnorm = t.cnorm.as<double>();
+  a.seeds = t.seeds.as<double4>();
+  a.G = G; a.sm = sm; a.sr = sr; a.alm = alm;
+  const int R = t.spin ? AdjCfg<true>::R : AdjCfg<false>::R;
+  const int maxw = (t.spin ? AdjCfg<true>::MaxThreads : AdjCfg<false>::MaxThreads) / 64;
+  int nw = std::min(maxw, ceil_div(t.nslot, 64 * R));  // more rings run as further passes
+  if (nw < 1) nw = 1;
+  const int na = t.spin ? 2 : 1;
+  const size_t lds = sizeof(double2) * (size_t)(L + 1) * na;
+  NUSHT_REQUIRE(lds <= 160 * 1024, kErrUnsupported, "legendre_adjoint: lmax too large for the LDS accumulator");
+  if (t.spin) {
+    NUSHT_HIP(hipFuncSetAttribute((const void*)k_adjoint<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_adjoint<true>, dim3(L + 1), dim3(64 * nw), lds, st, a);
+  } else {
+    NUSHT_HIP(hipFuncSetAttribute((const void*)k_adjoint<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    hipLaunchKernelGGL(k_adjoint<false>, dim3(L + 1), dim3(64 * nw), lds, st, a);
+  }
+  NUSHT_LAUNCH_CHECK();
+}
+
+}  // namespace nusht

@@ -1,0 +1,658 @@
+// Double-Fourier-sphere passes and ES nuFFT (see nufft.h).
+//
+// Kernel parameters follow the published ES-kernel method (Barnett, Magland & af Klinteberg,
+// SIAM J. Sci. Comput. 41 (2019)): for upsampling sigma, width w = ceil(ln(1/eps) / (pi sqrt(1 - 1/sigma))),
+// beta = 0.97 pi w (1 - 1/(2 sigma)); deconvolution by the kernel's Fourier transform, here
+// integrated with Gauss-Legendre quadrature on the host.  Point coordinates stay FP64 in every
+// precision: the integer cell and the fractional offset are formed in FP64 before the kernel
+// is evaluated in the working precision.
+#include "nufft.h"
+
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+
+namespace nusht {
+namespace {
+
+constexpr int kMaxW = 24;
+constexpr int kLdsBytes = 160 * 1024;
+
+template <typename T>
+__device__ __forceinline__ T es_kernel(T z, T beta) {
+  T t = T(1) - z * z;
+  return t > T(0) ? exp(beta * (sqrt(t) - T(1))) : T(0);
+}
+
+template <typename T>
+struct WorkT;  // working precision helpers
+template <>
+struct WorkT<float> {
+  __device__ static float ker(float z, float beta) {
+    float t = 1.0f - z * z;
+    return t > 0.0f ? __expf(beta * (__fsqrt_rn(t) - 1.0f)) : 0.0f;
+  }
+};
+template <>
+struct WorkT<double> {
+  __device__ static double ker(double z, double beta) {
+    double t = 1.0 - z * z;
+    return t > 0.0 ? exp(beta * (sqrt(t) - 1.0)) : 0.0;
+  }
+};
+
+// ------------------------------------------------------------------ theta pass, type 2
+template <typename T>
+struct ThetaArgs {
+  int lmax, spin, ntheta, ntd, nf, nm, row_lo, nrow, nmrows;
+  const double2* G;
+  long gsm;
+  const double* deconv;
+  FftDesc<T> ftd, fnf;
+  c2_t<T>* Tout;  // [nrow][nm]
+};
+
+template <typename T, int NT, int NMAX>
+__global__ __launch_bounds__(NT) void k_theta2(ThetaArgs<T> a) {
+  using T2 = c2_t<T>;
+  extern __shared__ unsigned char smem_raw[];
+  T2* buf = reinterpret_cast<T2*>(smem_raw);
+  const int row = xcd_remap(blockIdx.x, gridDim.x);
+  const int m = a.spin ? row - a.lmax : row;
+  const int am = m < 0 ? -m : m;
+  const bool neg = ((a.spin + am) & 1) != 0;
+  const double2* g = a.G + (long)row * a.gsm;
+  for (int r = threadIdx.x; r < a.ntd; r += NT) {
+    double2 v;
+    if (r < a.ntheta) v = g[r];
+    else { v = g[a.ntd - r]; if (neg) v = double2{-v.x, -v.y}; }
+    buf[r] = T2{(T)v.x, (T)v.y};
+  }
+  __syncthreads();
+  fft_lds<T, NT, NMAX>(buf, a.ftd);
+  // deconvolve, pad to nf (conjugated for the inverse transform)
+  constexpr int E = (NMAX + NT - 1) / NT;
+  T2 v[E];
+  const double fm = a.deconv[am] / a.ntd;
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    int k = threadIdx.x + i * NT;
+    v[i] = T2{T(0), T(0)};
+    if (k < a.nf) {
+      if (k <= a.lmax) {
+        T2 s = buf[k];
+        T f = (T)(fm * a.deconv[k]);
+        v[i] = T2{s.x * f, -s.y * f};
+      } else if (k >= a.nf - a.lmax) {
+        int kk = a.nf - k;
+        T2 s = buf[a.ntd - kk];
+        T f = (T)(fm * a.deconv[kk]);
+        v[i] = T2{s.x * f, -s.y * f};
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    int k = threadIdx.x + i * NT;
+    if (k < a.nf) buf[k] = v[i];
+  }
+  __syncthreads();
+  fft_lds<T, NT, NMAX>(buf, a.fnf);
+  // rows of the kept band; spin 0 also fills the conjugate column
+  for (int q = threadIdx.x; q < a.nrow; q += NT) {
+    int t = a.row_lo + q;
+    t %= a.nf;
+    if (t < 0) t += a.nf;
+    T2 s = buf[t];
+    T2 val{s.x, -s.y};
+    T2* orow = a.Tout + (long)q * a.nm;
+    orow[m + a.lmax] = val;
+    if (!a.spin && m > 0) orow[a.lmax - m] = T2{val.x, -val.y};
+  }
+}
+
+// ------------------------------------------------------------------ phi pass, type 2
+template <typename T>
+struct PhiArgs {
+  int lmax, nf, nm, nrow;
+  FftDesc<T> fnf;
+  c2_t<T>* Tbuf;  // [nrow][nm]
+  c2_t<T>* grid;  // [nrow][nf]
+};
+
+template <typename T, int NT, int NMAX>
+__global__ __launch_bounds__(NT) void k_phi2(PhiArgs<T> a) {
+  using T2 = c2_t<T>;
+  extern __shared__ unsigned char smem_raw[];
+  T2* buf = reinterpret_cast<T2*>(smem_raw);
+  const int q = blockIdx.x;
+  const T2* trow = a.Tbuf + (long)q * a.nm;
+  for (int k = threadIdx.x; k < a.nf; k += NT) {
+    int m = k <= a.lmax ? k : (k >= a.nf - a.lmax ? k - a.nf : 0x7fffffff);
+    T2 v{T(0), T(0)};
+    if (m != 0x7fffffff) { T2 s = trow[m + a.lmax]; v = T2{s.x, -s.y}; }
+    buf[k] = v;
+  }
+  __syncthreads();
+  fft_lds<T, NT, NMAX>(buf, a.fnf);
+  T2* grow = a.grid + (long)q * a.nf;
+  for (int k = threadIdx.x; k < a.nf; k += NT) {
+    T2 s = buf[k];
+    grow[k] = T2{s.x, -s.y};
+  }
+}
+
+// ------------------------------------------------------------------ phi pass, type 1 (adjoint)
+template <typename T, int NT, int NMAX>
+__global__ __launch_bounds__(NT) void k_phi1(PhiArgs<T> a) {
+  using T2 = c2_t<T>;
+  extern __shared__ unsigned char smem_raw[];
+  T2* buf = reinterpret_cast<T2*>(smem_raw);
+  const int q = blockIdx.x;
+  const T2* grow = a.grid + (long)q * a.nf;
+  for (int k = threadIdx.x; k < a.nf; k += NT) buf[k] = grow[k];
+  __syncthreads();
+  fft_lds<T, NT, NMAX>(buf, a.fnf);
+  T2* trow = a.Tbuf + (long)q * a.nm;
+  for (int i = threadIdx.x; i < a.nm; i += NT) {
+    int m = i - a.lmax;
+    int k = m >= 0 ? m : a.nf + m;
+    trow[i] = buf[k];
+  }
+}
+
+// ------------------------------------------------------------------ theta pass, type 1 (adjoint)
+template <typename T, int NT, int NMAX>
+__global__ __launch_bounds__(NT) void k_theta1(ThetaArgs<T> a) {
+  using T2 = c2_t<T>;
+  extern __shared__ unsigned char smem_raw[];
+  T2* buf = reinterpret_cast<T2*>(smem_raw);
+  const int row = xcd_remap(blockIdx.x, gridDim.x);
+  const int m = a.spin ? row - a.lmax : row;
+  const int am = m < 0 ? -m : m;
+  const bool neg = ((a.spin + am) & 1) != 0;
+  for (int k = threadIdx.x; k < a.nf; k += NT) buf[k] = T2{T(0), T(0)};
+  __syncthreads();
+  for (int q = threadIdx.x; q < a.nrow; q += NT) {
+    int t = a.row_lo + q;
+    t %= a.nf;
+    if (t < 0) t += a.nf;
+    buf[t] = a.Tout[(long)q * a.nm + m + a.lmax];
+  }
+  __syncthreads();
+  fft_lds<T, NT, NMAX>(buf, a.fnf);
+  constexpr int E = (NMAX + NT - 1) / NT;
+  T2 v[E];
+  const double fm = a.deconv[am] / a.ntd;
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    int r = threadIdx.x + i * NT;
+    v[i] = T2{T(0), T(0)};
+    if (r < a.ntd) {
+      if (r <= a.lmax) {
+        T2 s = buf[r];
+        T f = (T)(fm * a.deconv[r]);
+        v[i] = T2{s.x * f, -s.y * f};
+      } else if (r >= a.ntd - a.lmax) {
+        int kk = a.ntd - r;
+        T2 s = buf[a.nf - kk];
+        T f = (T)(fm * a.deconv[kk]);
+        v[i] = T2{s.x * f, -s.y * f};
+      }
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    int r = threadIdx.x + i * NT;
+    if (r < a.ntd) buf[r] = v[i];
+  }
+  __syncthreads();
+  fft_lds<T, NT, NMAX>(buf, a.ftd);
+  // fold back onto the ntheta rings (conjugate back from the inverse trick)
+  double2* g = const_cast<double2*>(a.G) + (long)row * a.gsm;
+  for (int j = threadIdx.x; j < a.ntheta; j += NT) {
+    T2 s = buf[j];
+    double re = (double)s.x, im = -(double)s.y;
+    if (j > 0 && j < a.ntheta - 1) {
+      T2 u = buf[a.ntd - j];
+      double ur = (double)u.x, ui = -(double)u.y;
+      if (neg) { re -= ur; im -= ui; } else { re += ur; im += ui; }
+    }
+    g[j] = double2{re, im};
+  }
+}
+
+// ------------------------------------------------------------------ interpolation (type 2)
+template <typename T>
+struct InterpArgs {
+  int nf, w, row_lo, nrow, spin;
+  T beta;
+  long npts;
+  const double2* pts;
+  const int* idx;
+  const c2_t<T>* grid;
+  const double* gamma;
+  double* out_re;
+  double* out_im;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_interp(InterpArgs<T> a) {
+  using T2 = c2_t<T>;
+  const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (p >= a.npts) return;
+  const double2 tp = a.pts[p];
+  const double scale = a.nf / (2.0 * M_PI);
+  const double hw = 0.5 * a.w;
+  const double ut = tp.x * scale;
+  double up = tp.y * scale;
+  up -= a.nf * floor(up / a.nf);
+  const int i0 = (int)ceil(ut - hw);
+  const int j0 = (int)ceil(up - hw);
+  const int q0 = i0 - a.row_lo;
+  const T inv = (T)(2.0 / a.w);
+  T kt[kMaxW], kp[kMaxW];
+  const double ft = i0 - ut, fp = j0 - up;
+#pragma unroll
+  for (int i = 0; i < kMaxW; ++i) {
+    if (i < a.w) {
+      kt[i] = WorkT<T>::ker((T)(ft + i) * inv, a.beta);
+      kp[i] = WorkT<T>::ker((T)(fp + i) * inv, a.beta);
+    }
+  }
+  T sr = 0, si = 0;
+  const bool inband = q0 >= 0 && q0 + a.w <= a.nrow;
+  if (inband) {
+    int jc = j0 < 0 ? j0 + a.nf : j0;
+    for (int i = 0; i < a.w; ++i) {
+      const T2* row = a.grid + (long)(q0 + i) * a.nf;
+      T rr = 0, ri = 0;
+      int c = jc;
+      for (int j = 0; j < a.w; ++j) {
+        T2 g = row[c];
+        rr = fma(g.x, kp[j], rr);
+        ri = fma(g.y, kp[j], ri);
+        ++c;
+        if (c == a.nf) c = 0;
+      }
+      sr = fma(rr, kt[i], sr);
+      si = fma(ri, kt[i], si);
+    }
+  }
+  const int o = a.idx[p];
+  double vr = inband ? (double)sr : __builtin_nan("");
+  double vi = inband ? (double)si : __builtin_nan("");
+  if (a.spin == 0) {
+    a.out_re[o] = vr;
+  } else {
+    if (a.gamma) {
+      double s, c;
+      sincos(a.spin * a.gamma[o], &s, &c);
+      double xr = vr * c - vi * s, xi = vr * s + vi * c;
+      vr = xr; vi = xi;
+    }
+    a.out_re[o] = vr;
+    a.out_im[o] = vi;
+  }
+}
+
+// ------------------------------------------------------------------ spreading (type 1)
+template <typename T>
+struct SpreadArgs {
+  int nf, w, row_lo, nrow, spin, tr, tc, ntc;
+  T beta;
+  const double2* pts;
+  const int* idx;
+  const int* tile_start;
+  const double* in_re;
+  const double* in_im;
+  const double* gamma;
+  c2_t<T>* grid;
+};
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_spread(SpreadArgs<T> a) {
+  using T2 = c2_t<T>;
+  extern __shared__ unsigned char smem_raw[];
+  T* tile = reinterpret_cast<T*>(smem_raw);  // interleaved re, im
+  const int tb = blockIdx.x;
+  const int s0 = a.tile_start[tb], s1 = a.tile_start[tb + 1];
+  if (s0 == s1) return;
+  const int tw = a.tc + a.w, th = a.tr + a.w;
+  const int ncell = tw * th;
+  for (int i = threadIdx.x; i < 2 * ncell; i += blockDim.x) tile[i] = T(0);
+  __syncthreads();
+  const int tile_r = tb / a.ntc, tile_c = tb % a.ntc;
+  const int r0 = tile_r * a.tr, c0 = tile_c * a.tc;
+  const double scale = a.nf / (2.0 * M_PI);
+  const double hw = 0.5 * a.w;
+  const T inv = (T)(2.0 / a.w);
+  for (int p = s0 + threadIdx.x; p < s1; p += blockDim.x) {
+    const double2 tp = a.pts[p];
+    const double ut = tp.x * scale;
+    double up = tp.y * scale;
+    up -= a.nf * floor(up / a.nf);
+    const int i0 = (int)ceil(ut - hw);
+    const int j0u = (int)ceil(up - hw);
+    const int j0 = j0u < 0 ? j0u + a.nf : j0u;
+    const int lr = i0 - a.row_lo - r0, lc = j0 - c0;
+    if (lr < 0 || lr >= a.tr || lc < 0 || lc >= a.tc) continue;  // never for points binned by set_pts
+    const int o = a.idx[p];
+    double vr = a.in_re[o], vi = (a.spin && a.in_im) ? a.in_im[o] : 0.0;
+    if (a.spin && a.gamma) {
+      double s, c;
+      sincos(-a.spin * a.gamma[o], &s, &c);
+      double xr = vr * c - vi * s, xi = vr * s + vi * c;
+      vr = xr; vi = xi;
+    }
+    T kp[kMaxW];
+    const double fp = j0u - up;
+    const double ft = i0 - ut;
+#pragma unroll
+    for (int j = 0; j < kMaxW; ++j)
+      if (j < a.w) kp[j] = WorkT<T>::ker((T)(fp + j) * inv, a.beta);
+    for (int i = 0; i < a.w; ++i) {
+      T kt = WorkT<T>::ker((T)(ft + i) * inv, a.beta);
+      T cr = (T)vr * kt, ci = (T)vi * kt;
+      T* trow = tile + 2 * ((lr + i) * tw + lc);
+      for (int j = 0; j < a.w; ++j) {
+        atomicAdd(&trow[2 * j], cr * kp[j]);
+        atomicAdd(&trow[2 * j + 1], ci * kp[j]);
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < ncell; i += blockDim.x) {
+    const int rr = i / tw, cc = i - rr * tw;
+    const int grow = r0 + rr;
+    if (grow >= a.nrow) continue;
+    T vr = tile[2 * i], vi = tile[2 * i + 1];
+    if (vr == T(0) && vi == T(0)) continue;
+    const int gc = (c0 + cc) % a.nf;  // a tile can be wider than a small grid
+    T* g = reinterpret_cast<T*>(a.grid + (long)grow * a.nf + gc);
+    atomicAdd(&g[0], vr);
+    atomicAdd(&g[1], vi);
+  }
+}
+
+// ------------------------------------------------------------------ point binning
+struct BinArgs {
+  int nf, w, row_lo, nrow, tr, tc, ntc, ntiles;
+  long npts;
+  const double* theta;
+  const double* phi;
+  int* key;
+  int* cnt;
+};
+
+__global__ void k_bin_count(BinArgs a, int* bad) {
+  const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (j >= a.npts) return;
+  const double scale = a.nf / (2.0 * M_PI);
+  const double hw = 0.5 * a.w;
+  const double ut = a.theta[j] * scale;
+  double up = a.phi[j] * scale;
+  up -= a.nf * floor(up / a.nf);
+  const int q0 = (int)ceil(ut - hw) - a.row_lo;
+  int j0 = (int)ceil(up - hw);
+  if (j0 < 0) j0 += a.nf;
+  if (j0 >= a.nf) j0 -= a.nf;
+  int k;
+  if (q0 < 0 || q0 + a.w > a.nrow || !(a.theta[j] == a.theta[j]) || !(a.phi[j] == a.phi[j])) {
+    atomicAdd(bad, 1);
+    k = 0;
+  } else {
+    k = (q0 / a.tr) * a.ntc + (j0 / a.tc);
+  }
+  a.key[j] = k;
+  atomicAdd(&a.cnt[k], 1);
+}
+
+__global__ void k_bin_scatter(long npts, const double* __restrict__ theta, const double* __restrict__ phi,
+                              const int* __restrict__ key, const int* __restrict__ start, int* cursor,
+                              double2* __restrict__ pts, int* __restrict__ idx) {
+  const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (j >= npts) return;
+  const int k = key[j];
+  const int pos = start[k] + atomicAdd(&cursor[k], 1);
+  pts[pos] = double2{theta[j], phi[j]};
+  idx[pos] = (int)j;
+}
+
+template <typename T>
+__global__ void k_fill_zero(T* p, long n) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = T(0);
+}
+
+// host Gauss-Legendre nodes on [-1, 1] (Newton on P_n), for the kernel's Fourier transform
+void gauss_legendre(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.resize(n);
+  w.resize(n);
+  for (int i = 0; i < n; ++i) {
+    double z = std::cos(M_PI * (i + 0.75) / (n + 0.5));
+    double pp = 0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = z;
+      for (int k = 2; k <= n; ++k) {
+        double p2 = ((2.0 * k - 1) * z * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      pp = n * (z * p1 - p0) / (z * z - 1.0);
+      double dz = p1 / pp;
+      z -= dz;
+      if (std::fabs(dz) < 1e-16) break;
+    }
+    x[i] = z;
+    w[i] = 2.0 / ((1.0 - z * z) * pp * pp);
+  }
+}
+
+template <typename T, int NT, int NMAX, typename K, typename A>
+void launch_pass(K kern, int nblocks, size_t lds, const A& args, hipStream_t st) {
+  NUSHT_HIP(hipFuncSetAttribute((const void*)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  hipLaunchKernelGGL(kern, dim3(nblocks), dim3(NT), lds, st, args);
+  NUSHT_LAUNCH_CHECK();
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ parameters
+NufftParams NufftParams::choose(int lmax, int spin, double eps, double sigma_hint) {
+  NUSHT_REQUIRE(lmax >= 1, kErrArg, "nufft: lmax must be >= 1");
+  NUSHT_REQUIRE(eps > 0 && eps < 1, kErrArg, "nufft: epsilon must be in (0, 1)");
+  NufftParams p;
+  p.lmax = lmax;
+  p.spin = spin;
+  p.eps = eps;
+  p.single = eps >= 1e-6;
+  const int maxlen = kLdsBytes / (p.single ? 8 : 16);
+  int ntd = fft_good_size(2 * lmax + 2);
+  while (ntd % 2) ntd = fft_good_size(ntd + 1);
+  NUSHT_REQUIRE(ntd <= maxlen, kErrUnsupported, "nufft: lmax too large for the in-LDS theta transform");
+  p.ntd = ntd;
+  p.ntheta = ntd / 2 + 1;
+  p.nm = 2 * lmax + 1;
+  double sigma = sigma_hint > 0 ? sigma_hint : (eps < 1e-8 ? 2.0 : 1.25);
+  int nf = fft_good_size((int)std::ceil(sigma * p.nm));
+  while (nf % 2) nf = fft_good_size(nf + 1);
+  if (nf > maxlen) {
+    nf = maxlen;
+    while (nf % 2 || !fft_supported(nf)) --nf;
+  }
+  p.nf = nf;
+  p.sigma = (double)nf / p.nm;
+  NUSHT_REQUIRE(p.sigma >= 1.2, kErrUnsupported, "nufft: fine grid too small for this lmax");
+  int w = (int)std::ceil(std::log(1.0 / eps) / (M_PI * std::sqrt(1.0 - 1.0 / p.sigma)));
+  w = std::max(w, 2);
+  NUSHT_REQUIRE(w <= kMaxW, kErrUnsupported, "nufft: kernel width exceeds the supported maximum");
+  p.w = w;
+  p.beta = 0.97 * M_PI * w * (1.0 - 1.0 / (2.0 * p.sigma));
+  // fine rows touched by points with theta in [0, pi] (one cell of slack on both sides)
+  int lo = -(w / 2) - 1;
+  int hi = (int)std::ceil(0.5 * nf - 0.5 * w) + w;
+  p.row_lo = lo;
+  p.nrow = std::min(nf, hi - lo + 1);
+  if (p.nrow == nf) p.row_lo = 0;
+  return p;
+}
+
+// ------------------------------------------------------------------ plan
+NufftPlan::NufftPlan(const NufftParams& p, long npts_max, hipStream_t st) : p_(p), npts_max_(npts_max) {
+  // deconvolution table: 1 / int_{-w/2}^{w/2} psi(2t/w) cos(2 pi k t / nf) dt
+  std::vector<double> xq, wq;
+  gauss_legendre(4 * p.w + 60, xq, wq);
+  std::vector<double> dec(p.lmax + 1);
+  for (int k = 0; k <= p.lmax; ++k) {
+    double s = 0;
+    for (size_t i = 0; i < xq.size(); ++i) {
+      double t = 0.5 * p.w * xq[i];
+      double z = xq[i];
+      double psi = std::exp(p.beta * (std::sqrt(std::max(0.0, 1.0 - z * z)) - 1.0));
+      s += wq[i] * psi * std::cos(2.0 * M_PI * k * t / p.nf);
+    }
+    dec[k] = 1.0 / (0.5 * p.w * s);
+  }
+  upload(deconv_, dec);
+  if (p.single) { ftd_f_.init(p.ntd); fnf_f_.init(p.nf); }
+  else { ftd_d_.init(p.ntd); fnf_d_.init(p.nf); }
+  const size_t cb = p.single ? 8 : 16;
+  T_.alloc(cb * (size_t)p.nrow * p.nm);
+  grid_.alloc(cb * (size_t)p.nrow * p.nf);
+  ntile_r_ = ceil_div(p.nrow, tile_r_);
+  ntile_c_ = ceil_div(p.nf, tile_c_);
+  const int ntiles = ntile_r_ * ntile_c_;
+  npts_max_ = std::max<long>(npts_max, 1);
+  NUSHT_REQUIRE(npts_max_ < (1L << 31), kErrUnsupported, "nufft: too many points");
+  pts_.alloc(sizeof(double2) * npts_max_);
+  idx_.alloc(sizeof(int) * npts_max_);
+  key_.alloc(sizeof(int) * npts_max_);
+  tile_start_.alloc(sizeof(int) * (ntiles + 1));
+  tile_cnt_.alloc(sizeof(int) * (ntiles + 2));
+  cursor_.alloc(sizeof(int) * (ntiles + 1));
+  NUSHT_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, scan_tmp_bytes_, tile_cnt_.as<int>(), tile_start_.as<int>(),
+                                             ntiles + 1, st));
+  scan_tmp_.alloc(std::max<size_t>(scan_tmp_bytes_, 16));
+}
+
+size_t NufftPlan::bytes() const {
+  return deconv_.bytes() + T_.bytes() + grid_.bytes() + pts_.bytes() + idx_.bytes() + key_.bytes() +
+         tile_start_.bytes() + tile_cnt_.bytes() + cursor_.bytes() + scan_tmp_.bytes();
+}
+
+void NufftPlan::set_pts(const double* theta, const double* phi, long npts, hipStream_t st) {
+  NUSHT_REQUIRE(npts >= 0 && npts <= npts_max_, kErrArg, "set_pts: npts exceeds the plan's maximum");
+  npts_ = npts;
+  const int ntiles = ntile_r_ * ntile_c_;
+  // tile_cnt_[ntiles + 1] is an error counter
+  hipLaunchKernelGGL(k_fill_zero<int>, dim3(ceil_div(ntiles + 2, 256)), dim3(256), 0, st, tile_cnt_.as<int>(),
+                     (long)ntiles + 2);
+  hipLaunchKernelGGL(k_fill_zero<int>, dim3(ceil_div(ntiles + 1, 256)), dim3(256), 0, st, cursor_.as<int>(),
+                     (long)ntiles + 1);
+  NUSHT_LAUNCH_CHECK();
+  if (npts == 0) {
+    hipLaunchKernelGGL(k_fill_zero<int>, dim3(ceil_div(ntiles + 1, 256)), dim3(256), 0, st, tile_start_.as<int>(),
+                       (long)ntiles + 1);
+    return;
+  }
+  BinArgs b{p_.nf, p_.w, p_.row_lo, p_.nrow, tile_r_, tile_c_, ntile_c_, ntiles, npts, theta, phi,
+            key_.as<int>(), tile_cnt_.as<int>()};
+  hipLaunchKernelGGL(k_bin_count, dim3(ceil_div(npts, 256)), dim3(256), 0, st, b, tile_cnt_.as<int>() + ntiles + 1);
+  NUSHT_LAUNCH_CHECK();
+  size_t tb = scan_tmp_bytes_;
+  NUSHT_HIP(hipcub::DeviceScan::ExclusiveSum(scan_tmp_.get(), tb, tile_cnt_.as<int>(), tile_start_.as<int>(),
+                                             ntiles + 1, st));
+  hipLaunchKernelGGL(k_bin_scatter, dim3(ceil_div(npts, 256)), dim3(256), 0, st, npts, theta, phi, key_.as<int>(),
+                     tile_start_.as<int>(), cursor_.as<int>(), pts_.as<double2>(), idx_.as<int>());
+  NUSHT_LAUNCH_CHECK();
+  int bad = 0;
+  NUSHT_HIP(hipMemcpyAsync(&bad, tile_cnt_.as<int>() + ntiles + 1, sizeof(int), hipMemcpyDeviceToHost, st));
+  NUSHT_HIP(hipStreamSynchronize(st));
+  if (bad != 0) {
+    npts_ = 0;  // the plan holds no valid points until the next successful set_pts
+    throw Error{kErrArg, "set_pts: " + std::to_string(bad) + " point(s) with theta outside [0, pi] or NaN"};
+  }
+}
+
+template <typename T>
+static FftDesc<T> desc_of(const FftLength<float>& f, const FftLength<double>& d);
+template <>
+FftDesc<float> desc_of<float>(const FftLength<float>& f, const FftLength<double>&) { return f.desc(); }
+template <>
+FftDesc<double> desc_of<double>(const FftLength<float>&, const FftLength<double>& d) { return d.desc(); }
+
+template <typename T, template <typename, int, int> class KT>
+struct Pass;
+
+template <typename T>
+static constexpr int nmax_big() { return kLdsBytes / (int)sizeof(c2_t<T>); }
+
+#define NUSHT_LAUNCH_SIZED(KERN, T, n, nblocks, lds, args, st)                                        \
+  do {                                                                                               \
+    if ((n) <= 2048) launch_pass<T, 256, 2048>(KERN<T, 256, 2048>, nblocks, lds, args, st);           \
+    else launch_pass<T, 512, nmax_big<T>()>(KERN<T, 512, nmax_big<T>()>, nblocks, lds, args, st);     \
+  } while (0)
+
+template <typename T>
+void NufftPlan::type2_impl(const double2* G, long sm, const double* gamma, double* ore, double* oim,
+                           hipStream_t st) {
+  using T2 = c2_t<T>;
+  const NufftParams& p = p_;
+  const FftDesc<T> ftd = desc_of<T>(ftd_f_, ftd_d_);
+  const FftDesc<T> fnf = desc_of<T>(fnf_f_, fnf_d_);
+  const int nmrows = p.spin ? 2 * p.lmax + 1 : p.lmax + 1;
+  ThetaArgs<T> ta{p.lmax, p.spin, p.ntheta, p.ntd, p.nf, p.nm, p.row_lo, p.nrow, nmrows, G, sm,
+                  deconv_.as<double>(), ftd, fnf, T_.as<T2>()};
+  const size_t lds = sizeof(T2) * (size_t)std::max(p.ntd, p.nf);
+  NUSHT_LAUNCH_SIZED(k_theta2, T, std::max(p.ntd, p.nf), nmrows, lds, ta, st);
+  PhiArgs<T> pa{p.lmax, p.nf, p.nm, p.nrow, fnf, T_.as<T2>(), grid_.as<T2>()};
+  NUSHT_LAUNCH_SIZED(k_phi2, T, p.nf, p.nrow, sizeof(T2) * (size_t)p.nf, pa, st);
+  if (npts_ == 0) return;
+  InterpArgs<T> ia{p.nf, p.w, p.row_lo, p.nrow, p.spin, (T)p.beta, npts_, pts_.as<double2>(), idx_.as<int>(),
+                   grid_.as<T2>(), gamma, ore, oim};
+  hipLaunchKernelGGL(k_interp<T>, dim3(ceil_div(npts_, 256)), dim3(256), 0, st, ia);
+  NUSHT_LAUNCH_CHECK();
+}
+
+template <typename T>
+void NufftPlan::type1_impl(const double* ire, const double* iim, const double* gamma, double2* G, long sm,
+                           hipStream_t st) {
+  using T2 = c2_t<T>;
+  const NufftParams& p = p_;
+  const FftDesc<T> ftd = desc_of<T>(ftd_f_, ftd_d_);
+  const FftDesc<T> fnf = desc_of<T>(fnf_f_, fnf_d_);
+  const long ngrid = 2L * p.nrow * p.nf;
+  hipLaunchKernelGGL(k_fill_zero<T>, dim3(std::min<long>(ceil_div(ngrid, 256), 65536)), dim3(256), 0, st,
+                     grid_.as<T>(), ngrid);
+  NUSHT_LAUNCH_CHECK();
+  if (npts_ > 0) {
+    SpreadArgs<T> sa{p.nf, p.w, p.row_lo, p.nrow, p.spin, tile_r_, tile_c_, ntile_c_, (T)p.beta, pts_.as<double2>(),
+                     idx_.as<int>(), tile_start_.as<int>(), ire, iim, gamma, grid_.as<T2>()};
+    const size_t lds = sizeof(T2) * (size_t)(tile_r_ + p.w) * (tile_c_ + p.w);
+    hipLaunchKernelGGL(k_spread<T>, dim3(ntile_r_ * ntile_c_), dim3(256), lds, st, sa);
+    NUSHT_LAUNCH_CHECK();
+  }
+  PhiArgs<T> pa{p.lmax, p.nf, p.nm, p.nrow, fnf, T_.as<T2>(), grid_.as<T2>()};
+  NUSHT_LAUNCH_SIZED(k_phi1, T, p.nf, p.nrow, sizeof(T2) * (size_t)p.nf, pa, st);
+  const int nmrows = p.spin ? 2 * p.lmax + 1 : p.lmax + 1;
+  ThetaArgs<T> ta{p.lmax, p.spin, p.ntheta, p.ntd, p.nf, p.nm, p.row_lo, p.nrow, nmrows, G, sm,
+                  deconv_.as<double>(), ftd, fnf, T_.as<T2>()};
+  const size_t lds = sizeof(T2) * (size_t)std::max(p.ntd, p.nf);
+  NUSHT_LAUNCH_SIZED(k_theta1, T, std::max(p.ntd, p.nf), nmrows, lds, ta, st);
+}
+
+void NufftPlan::type2(const double2* G, long sm, const double* gamma, double* ore, double* oim, hipStream_t st) {
+  if (p_.single) type2_impl<float>(G, sm, gamma, ore, oim, st);
+  else type2_impl<double>(G, sm, gamma, ore, oim, st);
+}
+
+void NufftPlan::type1(const double* ire, const double* iim, const double* gamma, double2* G, long sm,
+                      hipStream_t st) {
+  if (p_.single) type1_impl<float>(ire, iim, gamma, G, sm, st);
+  else type1_impl<double>(ire, iim, gamma, G, sm, st);
+}
+
+}  // namespace nusht

@@ -1,0 +1,47 @@
+// Stand-alone stage kernels (doubling, adjoint doubling, pointing) and the shared pointing math.
+#pragma once
+
+#include "common.h"
+
+namespace nusht {
+
+// Deflected angles for one pixel (cunusht/c/pointing.cu:205-223 formulas) and the
+// polarisation rotation angle (cunusht/deflection/CPU_nufft_transformer.py:158-160):
+//   a = |d|;  n1 = sin(t) cos(a) + cos(t) d_t sin(a)/a;  n2 = -d_p sin(a)/a;
+//   n3 = cos(t) cos(a) - sin(t) d_t sin(a)/a;  t' = atan2(sqrt(n1^2+n2^2), n3);
+//   p' = p - atan2(n2, n1), reduced by 2 pi only when >= 2 pi;
+//   gamma = atan2(d_p, d_t) - atan2(d_p, a sin(a) cot(t) + d_t cos(a)).
+__host__ __device__ inline void deflect_point(double t, double p, double dt, double dp, double& tp, double& pp,
+                                              double* gamma) {
+  const double a2 = dt * dt + dp * dp;
+  const double a = sqrt(a2);
+  double sa, ca;
+  if (a < 1e-3) {
+    sa = 1.0 - a2 / 6.0 * (1.0 - a2 / 20.0 * (1.0 - a2 / 42.0));
+    ca = 1.0 - a2 / 2.0 * (1.0 - a2 / 12.0 * (1.0 - a2 / 30.0 * (1.0 - a2 / 56.0)));
+  } else {
+    sa = sin(a) / a;
+    ca = cos(a);
+  }
+  const double st = sin(t), ct = cos(t);
+  const double n1 = st * ca + ct * dt * sa;
+  const double n2 = -dp * sa;
+  const double n3 = ct * ca - st * dt * sa;
+  tp = atan2(sqrt(n1 * n1 + n2 * n2), n3);
+  double q = p - atan2(n2, n1);
+  pp = (q >= 2.0 * M_PI) ? q - 2.0 * M_PI : q;
+  if (gamma) {
+    // a sin(a) = a^2 * sin(a)/a
+    *gamma = atan2(dp, dt) - atan2(dp, a2 * sa * ct / st + dt * ca);
+  }
+}
+
+template <typename T>
+void doubling(const T* map, int ntheta, int nphi, int spin, T* dmap, hipStream_t st);
+template <typename T>
+void adjoint_doubling(const T* dmap, int ntheta, int nphi, int spin, T* map, hipStream_t st);
+
+void pointing(const double* theta, const double* phi0, const long* nph, const long* ofs, int nring, long max_nph,
+              const double* dt, const double* dp, double* pt, double* pp, double* gamma, hipStream_t st);
+
+}  // namespace nusht

@@ -1,0 +1,156 @@
+/*
+ * nusht_amd.h -- C ABI of the MI355X (gfx950) nuSHT library  (libnusht_amd.so)
+ *
+ * Drop-in boundary for the GPU backend of cunuSHT.  Every pointer argument that names an array
+ * is a DEVICE pointer unless the comment says "host".  Every call takes an explicit stream
+ * (hipStream_t, NULL = default stream), enqueues its work and returns without synchronizing
+ * unless stated.  Every call returns NUSHT_OK (0) or an error code; nusht_last_error() gives
+ * the message of the calling thread's last failure.  No call allocates device memory except
+ * the *_create functions.
+ *
+ * Complex arrays are interleaved (re, im) doubles.  alm arrays use the healpy m-major layout
+ * idx(l, m) = m (2 lmax + 1 - m) / 2 + l with nalm = (lmax + 1)(lmax + 2) / 2 (mmax = lmax),
+ * one block of nalm per component (E then B for spin > 0).
+ *
+ * Conventions (orthonormal, Condon-Shortley): spin 0 f = sum a_lm Y_lm (real);
+ * spin s > 0: Q + iU = - sum (E_lm + i B_lm) sY_lm, returned as the two real arrays (Q, U).
+ */
+#ifndef NUSHT_AMD_H
+#define NUSHT_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* nusht_stream_t;                 /* hipStream_t */
+typedef struct nusht_plan_s* nusht_plan_t;     /* nuSHT plan: one (lmax, spin, epsilon) */
+typedef struct nusht_ringplan_s* nusht_ringplan_t; /* SHT on the iso-latitude pointing grid */
+
+#define NUSHT_OK 0
+#define NUSHT_ERR_ARG 1
+#define NUSHT_ERR_HIP 2
+#define NUSHT_ERR_UNSUPPORTED 3
+#define NUSHT_ERR_STATE 4
+#define NUSHT_ERR_NOMEM 5
+
+const char* nusht_last_error(void);
+int nusht_version(void);
+
+/* ---------------------------------------------------------------- stage kernels
+ * Theta doubling onto the double Fourier sphere.
+ * Replaces cunusht/c/doubling.cu:203-220 (CUdoubling_contig_1D, exported at :237-250):
+ *   map  (nphi, ntheta) theta-contiguous  ->  dmap (nphi, 2 ntheta - 2),
+ *   dmap[p][r] = map[p][r] (r < ntheta), (-1)^spin map[(p + nphi/2) % nphi][2 ntheta - 2 - r] otherwise
+ * (spin sign from cunusht/deflection/CPU_nufft_transformer.py:284-296).  nphi must be even. */
+int nusht_doubling_f64(const double* map, int ntheta, int nphi, int spin, double* dmap, nusht_stream_t s);
+int nusht_doubling_f32(const float* map, int ntheta, int nphi, int spin, float* dmap, nusht_stream_t s);
+int nusht_doubling_c128(const double* map, int ntheta, int nphi, int spin, double* dmap, nusht_stream_t s);
+
+/* Adjoint doubling.  Replaces cunusht/c/doubling.cu:75-92 (CUadjoint_doubling_1D, exported at :251-264):
+ *   dmap (2 ntheta - 2, nphi) ring-major -> map (ntheta, nphi),
+ *   map[r][p] = dmap[r][p] + (-1)^spin dmap[2 ntheta - 2 - r][(p + nphi/2) % nphi] for 0 < r < ntheta - 1
+ * (CPU_nufft_transformer.py:298-318). */
+int nusht_adjoint_doubling_f64(const double* dmap, int ntheta, int nphi, int spin, double* map, nusht_stream_t s);
+int nusht_adjoint_doubling_f32(const float* dmap, int ntheta, int nphi, int spin, float* map, nusht_stream_t s);
+int nusht_adjoint_doubling_c128(const double* dmap, int ntheta, int nphi, int spin, double* map, nusht_stream_t s);
+
+/* Deflected pointing.  Replaces cunusht/c/pointing.cu:228-255 (CUpointing_1Dto1D_lowmem, exported at
+ * :510-520) and adds the rotation angle gamma of CPU_nufft_transformer.py:158-160.
+ *   theta, phi0 (double[nring]), nph, ofs (int64[nring]): ring table; d_theta, d_phi (double[npix]):
+ *   spin-1 deflection; out_theta, out_phi (double[npix]); out_gamma may be NULL.
+ *   max_nph: host value, the largest entry of nph. */
+int nusht_pointing_f64(const double* theta, const double* phi0, const int64_t* nph, const int64_t* ofs,
+                       int nring, int64_t max_nph, const double* d_theta, const double* d_phi,
+                       double* out_theta, double* out_phi, double* out_gamma, nusht_stream_t s);
+
+/* ---------------------------------------------------------------- ring plan (pointing grid SHT)
+ * Replaces the GL-grid SHTns transformer of cunusht/sht/GPU_sht_transformer.py:28-34 and its
+ * synthesis_cupy (:56-62), adjoint_synthesis_cupy (:65-73, exact transpose, no weights) and
+ * synthesis_der1_cupy (:88-96) together with deflection.dlm2pointing
+ * (cunusht/deflection/GPU_nufft_transformer.py:64-104).
+ * theta, phi0, nph, ofs are HOST arrays of length nring; all rings must share one nph >= 2 lmax + 1. */
+typedef struct {
+  int lmax;
+  int spin;             /* 0: scalar maps, 1: deflection / pointing, 2: polarisation */
+  int nring;
+  const double* theta;  /* host */
+  const double* phi0;   /* host */
+  const int64_t* nph;   /* host */
+  const int64_t* ofs;   /* host */
+} nusht_ring_desc;
+
+int nusht_ringplan_create(const nusht_ring_desc* desc, nusht_stream_t s, nusht_ringplan_t* out);
+int nusht_ringplan_destroy(nusht_ringplan_t p);
+int64_t nusht_ringplan_npix(nusht_ringplan_t p);
+/* alm (ncomp x nalm) -> map0 (spin 0 map, or Q) and map1 (U; NULL for spin 0) */
+int nusht_ringplan_synthesis(nusht_ringplan_t p, const double* alm, int ncomp, double* map0, double* map1,
+                             nusht_stream_t s);
+/* exact adjoint of the synthesis */
+int nusht_ringplan_adjoint(nusht_ringplan_t p, const double* map0, const double* map1, double* alm, int ncomp,
+                           nusht_stream_t s);
+/* spin-1 plans: dlm_scaled = dlm / sqrt(l(l+1)) -> deflected (theta, phi) and gamma (may be NULL) */
+int nusht_ringplan_dlm2pointing(nusht_ringplan_t p, const double* dlm_scaled, double* ptg_theta, double* ptg_phi,
+                                double* gamma, nusht_stream_t s);
+
+/* ---------------------------------------------------------------- nuSHT plan
+ * Replaces GPU_cufinufft_transformer.__init__ / plan (cunusht/deflection/GPU_nufft_transformer.py:108-173,
+ * :227-246): geometry, recursion seeds, FFT twiddles, ES kernel and deconvolution tables and all
+ * work buffers are created here and stay resident until nusht_plan_destroy. */
+typedef struct {
+  int lmax;             /* mmax must equal lmax */
+  int mmax;
+  int spin;             /* 0, 1, 2 */
+  int ncomp;            /* 1 (spin 0 or GRAD_ONLY) or 2 (E, B) */
+  double epsilon;       /* target accuracy; FFT + nuFFT in FP32 when epsilon >= 1e-6 */
+  int64_t npts_max;     /* largest point set passed to set_pts */
+  double sigma;         /* fine-grid oversampling, 0 = automatic */
+} nusht_plan_desc;
+
+typedef struct {
+  int ntheta;           /* equiangular rings incl. poles used internally */
+  int ntheta_dfs;       /* 2 ntheta - 2 */
+  int nfine;            /* fine grid size per axis */
+  int nrow_fine;        /* fine theta rows actually formed */
+  int kernel_width;
+  int single_precision;
+  double sigma;
+  double beta;
+  int64_t workspace_bytes;
+} nusht_plan_info;
+
+int nusht_plan_create(const nusht_plan_desc* desc, nusht_stream_t s, nusht_plan_t* out);
+int nusht_plan_destroy(nusht_plan_t p);
+int nusht_plan_get_info(nusht_plan_t p, nusht_plan_info* info);
+
+/* Points (theta in [0, pi], any phi).  Replaces set_pts / cuFINUFFT setpts
+ * (GPU_nufft_transformer.py:249-251).  Synchronizes the stream once (bounds check). */
+int nusht_plan_set_pts(nusht_plan_t p, const double* theta, const double* phi, int64_t npts, nusht_stream_t s);
+
+/* alm -> values at the points.  Replaces nusht2d2 (GPU_nufft_transformer.py:386-435).
+ * spin 0: out0[npts] real; spin s: out0 = Q, out1 = U, rotated by exp(i s gamma) if gamma != NULL. */
+int nusht_nusht2d2(nusht_plan_t p, const double* alm, double* out0, double* out1, const double* gamma,
+                   nusht_stream_t s);
+/* exact adjoint.  Replaces nusht2d1 (GPU_nufft_transformer.py:437-484). */
+int nusht_nusht2d1(nusht_plan_t p, const double* in0, const double* in1, const double* gamma, double* alm,
+                   nusht_stream_t s);
+
+/* Lensing.  Replace gclm2lenmap (GPU_nufft_transformer.py:486-578) and lenmap2gclm (:580-673), with the
+ * spin-s rotation of the CPU backend (CPU_nufft_transformer.py:356-368, 429-529) when polrot != 0.
+ * ptg must be a spin-1 ring plan with the same lmax; its npix must not exceed npts_max. */
+int nusht_gclm2lenmap(nusht_plan_t p, nusht_ringplan_t ptg, const double* gclm, const double* dlm_scaled,
+                      double* lenmap0, double* lenmap1, int polrot, nusht_stream_t s);
+int nusht_lenmap2gclm(nusht_plan_t p, nusht_ringplan_t ptg, const double* lenmap0, const double* lenmap1,
+                      const double* dlm_scaled, double* gclm, int polrot, nusht_stream_t s);
+
+/* Batched synthesis: nmaps alm sets (stride alm_stride doubles between sets) on the plan's current
+ * points, sharing one Legendre recursion per (m, ring); outputs at out0 + k * out_stride (and out1). */
+int nusht_nusht2d2_batch(nusht_plan_t p, int nmaps, const double* alm, int64_t alm_stride, double* out0,
+                         double* out1, int64_t out_stride, const double* gamma, nusht_stream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NUSHT_AMD_H */

@@ -1,0 +1,252 @@
+"""GPU parity: the HIP library (through its C ABI) against the oracle on seeded inputs.
+
+Tolerances: index/byte work (doubling) bit-exact; FP64 paths (epsilon <= 1e-10) within 1e-9
+relative max error of the oracle's FP64 restatement or direct Y_lm summation; epsilon-controlled
+paths within a small multiple of epsilon (relative L2).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import almutil, clib, geometry as ogeom, pipeline, pointing as opointing, sht as osht
+
+pytestmark = pytest.mark.gpu
+
+
+_KEEP = []
+
+
+def _t(a):
+    """Device copy of a host array, kept alive until _sync() (a temporary's memory would otherwise be
+    recycled by the caching allocator before the kernel reads it)."""
+    t = torch.as_tensor(np.ascontiguousarray(a), device="cuda")
+    _KEEP.append(t)
+    return t
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _sync():
+    torch.cuda.synchronize()
+    _KEEP.clear()
+
+
+def _lib():
+    from cunusht_amd import _lib
+    return _lib
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _relmax(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def _rel2(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+# ------------------------------------------------------------------ doubling (bit-exact)
+@pytest.mark.parametrize("spin", [0, 1, 2])
+@pytest.mark.parametrize("ntheta,nphi", [(6, 8), (34, 64), (67, 130)])
+def test_doubling_exact(spin, ntheta, nphi):
+    rng = np.random.default_rng(ntheta + spin)
+    m = rng.standard_normal((nphi, ntheta))
+    out = torch.empty((nphi, 2 * ntheta - 2), dtype=torch.float64, device="cuda")
+    _lib().call("nusht_doubling_f64", _p(_t(m)), ntheta, nphi, spin, _p(out), _stream())
+    _sync()
+    # numpy restatement of doubling.cu:189-197 with the (-1)^s sign
+    ref = np.empty((nphi, 2 * ntheta - 2))
+    ref[:, :ntheta] = m
+    for r in range(ntheta, 2 * ntheta - 2):
+        ref[:, r] = np.roll(m, -nphi // 2, axis=0)[:, 2 * ntheta - 2 - r] * (-1) ** spin
+    assert np.array_equal(out.cpu().numpy(), ref)
+    mc = rng.standard_normal((nphi, ntheta)) + 1j * rng.standard_normal((nphi, ntheta))
+    outc = torch.empty((nphi, 2 * ntheta - 2), dtype=torch.complex128, device="cuda")
+    _lib().call("nusht_doubling_c128", _p(_t(mc)), ntheta, nphi, spin, _p(outc), _stream())
+    _sync()
+    refc = np.empty((nphi, 2 * ntheta - 2), dtype=complex)
+    refc[:, :ntheta] = mc
+    for r in range(ntheta, 2 * ntheta - 2):
+        refc[:, r] = np.roll(mc, -nphi // 2, axis=0)[:, 2 * ntheta - 2 - r] * (-1) ** spin
+    assert np.array_equal(outc.cpu().numpy(), refc)
+
+
+@pytest.mark.parametrize("spin", [0, 1, 2])
+@pytest.mark.parametrize("ntheta,nphi", [(6, 8), (34, 64), (67, 130)])
+def test_adjoint_doubling_exact(spin, ntheta, nphi):
+    rng = np.random.default_rng(100 + ntheta + spin)
+    d = rng.standard_normal((2 * ntheta - 2, nphi)).astype(np.float32)
+    out = torch.empty((ntheta, nphi), dtype=torch.float32, device="cuda")
+    _lib().call("nusht_adjoint_doubling_f32", _p(_t(d)), ntheta, nphi, spin, _p(out), _stream())
+    _sync()
+    # restatement of doubling.cu:61-69 / CPU_nufft_transformer.py:303-308
+    ref = d[:ntheta].copy()
+    h = nphi // 2
+    sg = np.float32(-1.0 if spin % 2 else 1.0)
+    ref[1:ntheta - 1, :h] += sg * d[-1:ntheta - 1:-1, h:]
+    ref[1:ntheta - 1, h:] += sg * d[-1:ntheta - 1:-1, :h]
+    assert np.array_equal(out.cpu().numpy(), ref)
+
+
+def test_doubling_pair_is_adjoint():
+    rng = np.random.default_rng(5)
+    nt, nph = 19, 36
+    m = rng.standard_normal((nt, nph))       # ring-major map
+    d = rng.standard_normal((2 * nt - 2, nph))
+    # ring-major doubling == transpose of the theta-contiguous kernel
+    dm = torch.empty((nph, 2 * nt - 2), dtype=torch.float64, device="cuda")
+    _lib().call("nusht_doubling_f64", _p(_t(m.T)), nt, nph, 2, _p(dm), _stream())
+    am = torch.empty((nt, nph), dtype=torch.float64, device="cuda")
+    _lib().call("nusht_adjoint_doubling_f64", _p(_t(d)), nt, nph, 2, _p(am), _stream())
+    _sync()
+    lhs = np.sum(d * dm.cpu().numpy().T)
+    rhs = np.sum(m * am.cpu().numpy())
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
+# ------------------------------------------------------------------ pointing
+def test_pointing_kernel():
+    rng = np.random.default_rng(9)
+    L = 31
+    geom = ogeom.gl_geometry(L)
+    th, ph = geom.pixel_angles()
+    dt = rng.normal(0, 1e-3, th.size)
+    dp = rng.normal(0, 1e-3, th.size)
+    dt[::7] *= 400  # some large deflections take the exact sin(a)/a branch
+    o = [torch.empty(th.size, dtype=torch.float64, device="cuda") for _ in range(3)]
+    _lib().call("nusht_pointing_f64", _p(_t(geom.theta)), _p(_t(geom.phi0)), _p(_t(geom.nph)), _p(_t(geom.ofs)),
+                geom.theta.size, int(geom.nph.max()), _p(_t(dt)), _p(_t(dp)), _p(o[0]), _p(o[1]), _p(o[2]),
+                _stream())
+    _sync()
+    tp, pp = opointing.deflected_angles(th, ph, dt, dp)
+    g = opointing.gamma(th, dt, dp)
+    # test/test_pointingGPU.py:121-122 tolerances (std of differences)
+    assert np.std(o[0].cpu().numpy() - tp) <= 1e-15
+    dphi = (o[1].cpu().numpy() - pp + np.pi) % (2 * np.pi) - np.pi
+    assert np.std(dphi) <= 1e-12
+    assert np.max(np.abs(o[2].cpu().numpy() - g)) < 1e-12
+
+
+# ------------------------------------------------------------------ ring SHT on the pointing grid
+@pytest.mark.parametrize("L,nphi", [(16, 33), (12, 25), (20, 48), (11, 23), (24, 49), (8, 17)])
+@pytest.mark.parametrize("spin", [0, 1, 2])
+def test_ring_synthesis_and_adjoint(L, nphi, spin):
+    """nphi covers the direct (25, 48, 49), Rader (17) and Bluestein (23, 33) ring DFTs."""
+    import cunusht_amd
+    rng = np.random.default_rng(L * 10 + spin)
+    T = cunusht_amd.get_transformer("GPU")(("gl", {"lmax": L, "nphi": nphi}))
+    nc = 1 if spin == 0 else 2
+    a = almutil.random_alm(L, spin, nc, rng)
+    ain = a[0] if spin == 0 else a
+    out = T.synthesis(_t(ain), lmax=L, mmax=L, spin=spin)
+    torch.cuda.synchronize()
+    rings = ogeom.gl_geometry(L, nphi=nphi)
+    th, ph = rings.pixel_angles()
+    ref = osht.synth_direct(ain, L, spin, th, ph)
+    got = out.cpu().numpy()
+    if spin:
+        ref = np.array([ref.real, ref.imag])
+    assert _relmax(got, ref) < 1e-12
+    x = rng.standard_normal(got.shape)
+    ad = T.adjoint_synthesis(_t(x), lmax=L, mmax=L, spin=spin, ncomp=nc).cpu().numpy().reshape(nc, -1)
+    lhs = float(np.sum(x * got))
+    rhs = sum(almutil.myalmdot(a[c], ad[c], L) for c in range(nc))
+    assert abs(lhs - rhs) < 1e-11 * abs(lhs)
+
+
+@pytest.mark.parametrize("L,nphi", [(40, 81), (33, 67), (31, 64)])
+def test_dlm2pointing_vs_oracle(L, nphi):
+    import cunusht_amd
+    rng = np.random.default_rng(L)
+    ll = np.arange(L + 1)
+    dlm = almutil.synalm(np.where(ll >= 2, 1e-5 / (ll + 1.0) ** 2, 0.0), L, rng)
+    T = cunusht_amd.get_transformer("GPU")(("gl", {"lmax": L, "nphi": nphi}))
+    got = T.dlm2pointing(_t(dlm), gamma=True).cpu().numpy()
+    rings = ogeom.gl_geometry(L, nphi=nphi)
+    tp, pp, g, _ = pipeline.dlm2pointing(dlm, L, rings)
+    assert np.max(np.abs(got[0] - tp)) < 1e-14
+    assert np.max(np.abs((got[1] - pp + np.pi) % (2 * np.pi) - np.pi)) < 1e-13
+    assert np.max(np.abs(got[2] - g)) < 1e-10
+
+
+# ------------------------------------------------------------------ nusht2d2 / nusht2d1
+@pytest.mark.parametrize("spin", [0, 2])
+@pytest.mark.parametrize("eps", [1e-12, 1e-7, 1e-5])
+def test_nusht2d2_vs_direct(spin, eps):
+    import cunusht_amd
+    rng = np.random.default_rng(int(-np.log10(eps)) + spin)
+    L = 30
+    npts = 3000
+    theta = np.arccos(rng.uniform(-1, 1, npts))
+    phi = rng.uniform(0, 2 * np.pi, npts)
+    loc = np.stack([theta, phi], 1)
+    nc = 1 if spin == 0 else 2
+    a = almutil.random_alm(L, spin, nc, rng)
+    ain = a[0] if spin == 0 else a
+    T = cunusht_amd.get_transformer("GPU")(("gl", {"lmax": L}), nuFFTtype=2, epsilon=eps, spin=spin)
+    got = T.nusht2d2(L, L, _t(ain), _t(loc), spin=spin).cpu().numpy()
+    ref = osht.synth_direct(ain, L, spin, theta, phi)
+    if spin:
+        ref = np.array([ref.real, ref.imag])
+    err = _rel2(got, ref)
+    assert err < 10 * max(eps, 1e-13), err
+
+
+@pytest.mark.parametrize("spin", [0, 2])
+@pytest.mark.parametrize("eps", [1e-10, 1e-5])
+def test_nusht2d1_adjointness(spin, eps):
+    """Dot-product test <x, A y> = <A^H x, y> (test/test_adjointOperator.py:247-277, rtol 1e-9)."""
+    import cunusht_amd
+    rng = np.random.default_rng(48)
+    L = 40
+    npts = 4000
+    loc = np.stack([np.pi * rng.uniform(0, 1, npts), 2 * np.pi * rng.uniform(0, 1, npts)], 1)
+    nc = 1 if spin == 0 else 2
+    a = almutil.random_alm(L, spin, nc, rng)
+    ain = a[0] if spin == 0 else a
+    x = rng.uniform(-0.5, 0.5, (nc, npts))
+    xin = x[0] if spin == 0 else x
+    T = cunusht_amd.get_transformer("GPU")(("gl", {"lmax": L}), nuFFTtype=1, epsilon=eps, spin=spin)
+    y = T.nusht2d2(L, L, _t(ain), _t(loc), spin=spin).cpu().numpy().reshape(nc, -1)
+    ad = T.nusht2d1(L, L, _t(xin), _t(loc), spin=spin).cpu().numpy().reshape(nc, -1)
+    lhs = float(np.sum(x * y))
+    rhs = sum(almutil.myalmdot(a[c], ad[c], L) for c in range(nc))
+    tol = 1e-9 if eps < 1e-6 else 1e-5
+    assert abs(lhs - rhs) < tol * abs(lhs), (lhs, rhs)
+    # and against the oracle's adjoint
+    vals = xin if spin == 0 else (x[0] + 1j * x[1])
+    ref = osht.adjoint_direct(vals, L, spin, loc[:, 0], loc[:, 1]).reshape(nc, -1)
+    assert _rel2(ad, ref) < 10 * max(eps, 1e-12)
+
+
+# ------------------------------------------------------------------ lensing
+@pytest.mark.parametrize("spin", [0, 2])
+def test_gclm2lenmap_vs_oracle(spin):
+    import cunusht_amd
+    rng = np.random.default_rng(77 + spin)
+    L = 48
+    nphi = 2 * L + 1  # 97: prime -> Rader
+    ll = np.arange(L + 1)
+    dlm = almutil.synalm(np.where(ll >= 2, 1e-4 / (ll + 1.0) ** 2, 0.0), L, rng)
+    nc = 1 if spin == 0 else 2
+    g = almutil.random_alm(L, spin, nc, rng)
+    gin = g[0] if spin == 0 else g
+    T = cunusht_amd.get_transformer("GPU")(("gl", {"lmax": L, "nphi": nphi}), nuFFTtype=2, epsilon=1e-12,
+                                           spin=spin)
+    got = T.gclm2lenmap(_t(gin), L, L, dlm_scaled=_t(dlm), spin=spin).cpu().numpy()
+    rings = ogeom.gl_geometry(L, nphi=nphi)
+    ref = pipeline.gclm2lenmap(gin, dlm, L, spin, rings, 1e-13)
+    assert _relmax(got, ref) < 1e-9
+    x = rng.standard_normal(got.shape)
+    ad = T.lenmap2gclm(_t(x), _t(dlm), None, L, L, spin=spin).cpu().numpy().reshape(nc, -1)
+    refad = pipeline.lenmap2gclm(x, dlm, L, spin, nc, rings, 1e-13)
+    assert _relmax(ad, refad) < 1e-9
