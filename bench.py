@@ -1,0 +1,283 @@
+"""Benchmark: spin-2 lensed maps/s at lmax 4095, eps 1e-7 on one MI355X.
+
+Workload (synthetic, seeded): E alm with C_l ~ exp(-(l/3000)^2) / (l(l+1)), B with 0.1 times that
+spectrum; deflection with C_l^dd ~ l^-2 (2 <= l <= lmax) normalised to 2.5 arcmin rms; pointing grid
+GL (lmax+1) x (2 lmax + 1).  All inputs are resident in HBM before the timed region.
+
+Two paths are timed:
+  plain   : one gclm2lenmap per step, each with its own deflection (spin-1 SHT + pointing,
+            point binning, spin-2 nuSHT with the polarisation rotation);
+  batched : B maps per step sharing one deflection (pointing once per step, one Legendre
+            recursion for all B maps, then the per-map DFS / nuFFT passes).
+`value` is the batched rate (lensed maps/s); the plain rate, p50/p99 per-call latency, the
+roofline of the dominant kernel (live hipEvent timing inside the library) and an oracle CPU
+baseline are reported beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--lmax L] [--eps E] [--batch B] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+PEAK_FP64_TFLOPS = 78.6
+
+
+def synth_inputs(lmax: int, rng: np.random.Generator):
+    from cunusht_amd.utils import almxfl, synalm
+    ll = np.arange(lmax + 1, dtype=np.float64)
+    clee = np.zeros(lmax + 1)
+    clee[2:] = np.exp(-(ll[2:] / 3000.0) ** 2) / (ll[2:] * (ll[2:] + 1)) * 1e3
+    elm = synalm(clee, lmax, rng=rng)
+    blm = synalm(0.1 * clee, lmax, rng=rng)
+    cldd = np.zeros(lmax + 1)
+    cldd[2:] = ll[2:] ** -2.0
+    rms = math.sqrt(np.sum((2 * ll + 1) * cldd) / (4 * math.pi))
+    cldd *= (2.5 / 60.0 * math.pi / 180.0 / rms) ** 2
+    dlm = synalm(cldd, lmax, rng=rng)
+    with np.errstate(divide="ignore"):
+        fl = np.where(ll > 0, 1.0 / np.sqrt(ll * (ll + 1)), 0.0)
+    dlm_scaled = almxfl(dlm, fl, lmax, False)
+    return np.stack([elm, blm]), dlm_scaled
+
+
+def stage_work(stage: str, info: dict, lmax: int, npix: int, maps_per_launch: int = 1):
+    """(flops or bytes, unit) of one launch of a stage -- the algorithmic work defined in DESIGN.md.
+
+    Spin-s Legendre synthesis: per (l, m, ring pair) 7 flops of recursion (shared by the maps of one
+    launch) plus 16 flops of contraction per map (4 complex-by-real FMAs: +m/-m, north/south)."""
+    nlm = (lmax + 1) * (lmax + 2) // 2
+    nt = info["ntheta"]
+    single = info["single_precision"]
+    bc = 8 if single else 16
+    if stage == "legendre_synth_cc_s2":
+        return nlm * math.ceil(nt / 2) * (7 + 16 * maps_per_launch), "flop"
+    if stage == "legendre_synth_ring_s1":
+        return nlm * math.ceil((lmax + 1) / 2) * (7 + 16), "flop"
+    if stage == "nufft_interp":
+        return npix * (16 + 16) + info["nrow_fine"] * info["nfine"] * bc, "byte"
+    if stage == "theta_pass_t2":
+        return (2 * lmax + 1) * nt * 16 + info["nrow_fine"] * (2 * lmax + 1) * bc, "byte"
+    if stage == "phi_pass_t2":
+        return info["nrow_fine"] * ((2 * lmax + 1) + info["nfine"]) * bc, "byte"
+    if stage == "ring_dft_pointing":
+        return (lmax + 1) * (2 * lmax + 1) * 16 + npix * 24, "byte"
+    return None, None
+
+
+def cpu_baseline(lmax_sample: int, lmax: int, eps: float):
+    """Oracle (C/OpenMP FP64 restatement of the reference CPU algorithm) on a bounded sample:
+    one spin-2 gclm2lenmap at lmax_sample; Legendre stages scaled to `lmax` by N_lm * nrings,
+    the other stages by the pixel count."""
+    from oracle import clib, geometry as ogeom, pipeline
+    from oracle.almutil import almxfl
+    rng = np.random.default_rng(2024)
+    gclm, dlm_scaled = synth_inputs(lmax_sample, rng)
+    rings = ogeom.gl_geometry(lmax_sample, nphi=2 * lmax_sample + 1)
+    L = lmax_sample
+    t0 = time.perf_counter()
+    ll = np.arange(L + 1, dtype=np.float64)
+    dlm = almxfl(dlm_scaled, np.sqrt(ll * (ll + 1)), L)
+    ta = time.perf_counter()
+    clib.sht_synth_rings(dlm[None, :], L, 1, rings.theta)
+    t_leg1 = time.perf_counter() - ta
+    op = pipeline.NuSHT(L)
+    ta = time.perf_counter()
+    clib.sht_synth_rings(gclm, L, 2, op.thetas)
+    t_leg2 = time.perf_counter() - ta
+    pipeline.gclm2lenmap(gclm, dlm_scaled, L, 2, rings, eps)
+    t_total = time.perf_counter() - t0 - t_leg1 - t_leg2  # full chain (includes one pass of each Legendre)
+    t_leg = t_leg1 + t_leg2
+    t_rest = max(t_total - t_leg, 0.0)
+    nlm_s, nlm = (L + 1) * (L + 2) // 2, (lmax + 1) * (lmax + 2) // 2
+    nt_s = geometry_ntheta_ref(L)
+    nt = geometry_ntheta_ref(lmax)
+    f_leg = (nlm * (lmax + 1 + nt)) / (nlm_s * (L + 1 + nt_s))
+    f_rest = ((lmax + 1) * (2 * lmax + 1)) / ((L + 1) * (2 * L + 1))
+    t_scaled = t_leg * f_leg + t_rest * f_rest
+    return {"value": 1.0 / t_scaled, "unit": "lensed maps/s", "cores": len(os.sched_getaffinity(0)),
+            "kind": "port",
+            "sample": f"oracle gclm2lenmap spin-2 lmax={L} eps={eps:g} (1 map, {t_leg + t_rest:.1f} s: "
+                      f"Legendre {t_leg:.1f} s scaled x{f_leg:.1f}, rest {t_rest:.1f} s scaled x{f_rest:.1f})"}
+
+
+def geometry_ntheta_ref(lmax: int) -> int:
+    from oracle.geometry import reference_cc_sizes
+    return reference_cc_sizes(lmax)[0]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--lmax", type=int, default=4095)
+    ap.add_argument("--eps", type=float, default=1e-7)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-lmax", type=int, default=1023)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every box in this setup has a single MI355X; "
+              "requests shard across GPUs without communication (see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import cunusht_amd
+    from cunusht_amd import _lib
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    L, eps, B = args.lmax, args.eps, args.batch
+    print(f"[bench] lmax={L} eps={eps:g} batch={B} steps={args.steps} warmup={args.warmup}", file=sys.stderr,
+          flush=True)
+    rng = np.random.default_rng(1234)
+    nin = 2
+    gclms, dlms = [], []
+    for i in range(max(nin, 1)):
+        g, d = synth_inputs(L, rng)
+        gclms.append(torch.as_tensor(g, device="cuda"))
+        dlms.append(torch.as_tensor(d, device="cuda"))
+    gb = torch.stack([gclms[i % nin] for i in range(B)]).contiguous()
+    print("[bench] inputs ready", file=sys.stderr, flush=True)
+
+    geominfo = ("gl", {"lmax": L, "nphi": 2 * L + 1})
+    T = cunusht_amd.get_transformer("GPU")(geominfo, nuFFTtype=2, epsilon=eps, spin=2)
+    npix = T.npix()
+    out = torch.empty((2, npix), dtype=torch.float64, device="cuda")
+
+    def plain_step(i):
+        T.gclm2lenmap(gclms[i % nin], L, L, dlm_scaled=dlms[i % nin], lenmap=out, spin=2)
+
+    # batched: B maps, one shared deflection per step (pointing once, one recursion for all B)
+    plan = T._plan_for(2, 2, eps, npix)
+    ring = T._ring(1)
+    ptg = torch.empty((3, npix), dtype=torch.float64, device="cuda")
+    outb = torch.empty((B, 2, npix), dtype=torch.float64, device="cuda")
+    stream = lambda: __import__("ctypes").c_void_p(torch.cuda.current_stream().cuda_stream)
+    P = lambda t: __import__("ctypes").c_void_p(t.data_ptr())
+    nalm2 = gb.shape[1] * gb.shape[2] * 2  # doubles per map (2 components, complex)
+    nb_done = 0
+
+    def batched_step(i):
+        d = dlms[i % nin]
+        _lib.call("nusht_ringplan_dlm2pointing", ring._h, P(d), P(ptg[0]), P(ptg[1]), P(ptg[2]), stream())
+        _lib.call("nusht_plan_set_pts", plan._h, P(ptg[0]), P(ptg[1]), npix, stream())
+        plan.pts_key = None
+        done = 0
+        while done < B:
+            nb = min(8, B - done)
+            _lib.call("nusht_nusht2d2_batch", plan._h, nb, P(gb[done]), nalm2, P(outb[done, 0]), P(outb[done, 1]),
+                      2 * npix, P(ptg[2]), stream())
+            done += nb
+
+    # warmup (also builds every plan and table)
+    for i in range(args.warmup):
+        plain_step(i)
+        batched_step(i)
+    torch.cuda.synchronize()
+    print("[bench] warmup done", file=sys.stderr, flush=True)
+    info = plan.info()
+
+    # plain: per-call latency with events on the launch stream
+    _lib.timing_reset()
+    _lib.timing_enable(True)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        evs[i][0].record()
+        plain_step(i + 1)
+        evs[i][1].record()
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    stages_plain = _lib.timing_read()
+    lat = np.array([a.elapsed_time(b) for a, b in evs])
+    print(f"[bench] plain: {args.steps / t_plain:.2f} maps/s", file=sys.stderr, flush=True)
+
+    # batched
+    _lib.timing_reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        batched_step(i + 1)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    stages_batch = _lib.timing_read()
+    _lib.timing_enable(False)
+    rate_batch = args.steps * B / t_batch
+    rate_plain = args.steps / t_plain
+    print(f"[bench] batched: {rate_batch:.2f} maps/s", file=sys.stderr, flush=True)
+
+    # roofline of the dominant stage of the headline (batched) path
+    dom = max(stages_batch.items(), key=lambda kv: kv[1][0])
+    name, (tot_ms, calls) = dom
+    launches_cc = stages_batch.get("legendre_synth_cc_s2", (0, 1))[1]
+    mpl = B * args.steps / max(launches_cc, 1) if name == "legendre_synth_cc_s2" else 1
+    work, unit = stage_work(name, info, L, npix, mpl)
+    roof = None
+    if work is not None:
+        avg_s = tot_ms / calls / 1e3
+        if unit == "flop":
+            ach = work / avg_s / 1e12
+            roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                    "frac": ach / PEAK_FP64_TFLOPS, "traffic": None, "kernel": name, "avg_ms": tot_ms / calls,
+                    "maps_per_launch": mpl}
+        else:
+            ach = work / avg_s / 1e9
+            roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                    "frac": ach / PEAK_HBM_GBS, "traffic": None, "kernel": name, "avg_ms": tot_ms / calls}
+    # the nuFFT interpolation against the HBM roofline
+    interp = stages_batch.get("nufft_interp")
+    nufft_roof = None
+    if interp:
+        w_i, _ = stage_work("nufft_interp", info, L, npix)
+        a = w_i / (interp[0] / interp[1] / 1e3) / 1e9
+        nufft_roof = {"achieved_GBs": a, "frac": a / PEAK_HBM_GBS, "avg_ms": interp[0] / interp[1]}
+
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(args.cpu_lmax, L, eps)
+
+    res = {
+        "metric": "spin-2 lensed maps/s at lmax 4095, eps 1e-7 (served vs plain); p50/p99 ms",
+        "value": rate_batch,
+        "unit": "lensed maps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_batch / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if info["single_precision"] else "f64",
+        "data": "synthetic (seeded E/B alm and deflection, GL 4096x8191 pointing grid)",
+        "config": {"workload": f"spin-2 gclm2lenmap lmax={L} eps={eps:g}, GL {L + 1}x{2 * L + 1}, "
+                               f"batched {B} maps per shared deflection",
+                   "lmax": L, "epsilon": eps, "batch": B, "npix": npix,
+                   "nufft": {"nfine": info["nfine"], "nrow": info["nrow_fine"], "w": info["kernel_width"],
+                             "sigma": round(info["sigma"], 4)}},
+        "plain": {"value": rate_plain, "p50_ms": float(np.percentile(lat, 50)),
+                  "p99_ms": float(np.percentile(lat, 99))},
+        "served_over_plain": rate_batch / rate_plain,
+        "roofline": roof,
+        "nufft_interp_roofline": nufft_roof,
+        "stages_ms_per_step": {k: v[0] / args.steps for k, v in sorted(stages_batch.items())},
+        "stages_plain_ms_per_call": {k: v[0] / args.steps for k, v in sorted(stages_plain.items())},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

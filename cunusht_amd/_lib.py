@@ -69,6 +69,10 @@ SIGNATURES = {
     "nusht_gclm2lenmap": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _vp],
     "nusht_lenmap2gclm": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _vp],
     "nusht_nusht2d2_batch": [_vp, _i, _vp, _i64, _vp, _vp, _i64, _vp, _vp],
+    "nusht_timing_enable": [_i],
+    "nusht_timing_reset": [],
+    "nusht_timing_count": [],
+    "nusht_timing_get": [_i, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(_d), ctypes.POINTER(_i64)],
 }
 _RESTYPES = {"nusht_last_error": ctypes.c_char_p, "nusht_ringplan_npix": _i64}
 
@@ -99,3 +103,27 @@ def check(code: int) -> None:
 
 def call(name: str, *args) -> None:
     check(getattr(load(), name)(*args))
+
+
+def timing_enable(on: bool = True) -> None:
+    call("nusht_timing_enable", 1 if on else 0)
+
+
+def timing_reset() -> None:
+    call("nusht_timing_reset")
+
+
+def timing_read() -> dict:
+    """{stage: (total_ms, calls)} accumulated since the last reset (synchronizes on the timing events)."""
+    lib = load()
+    n = lib.nusht_timing_count()
+    if n < 0:
+        check(-n)
+    out = {}
+    for i in range(n):
+        nm = ctypes.c_char_p()
+        ms = _d()
+        c = _i64()
+        call("nusht_timing_get", i, ctypes.byref(nm), ctypes.byref(ms), ctypes.byref(c))
+        out[nm.value.decode()] = (ms.value, c.value)
+    return out

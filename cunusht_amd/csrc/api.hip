@@ -9,6 +9,7 @@
 #include "nufft.h"
 #include "ring.h"
 #include "stages.h"
+#include "timing.h"
 
 using namespace nusht;
 
@@ -152,6 +153,7 @@ int nusht_plan_create(const nusht_plan_desc* d, nusht_stream_t s, nusht_plan_t* 
     p->np = NufftParams::choose(d->lmax, d->spin, d->epsilon, d->sigma);
     std::vector<double> th(p->np.ntheta);
     for (int j = 0; j < p->np.ntheta; ++j) th[j] = M_PI * j / (p->np.ntheta - 1);
+    p->tab.tag = "cc";
     p->tab.build(d->lmax, d->spin, th, S(s));
     p->nu = std::make_unique<NufftPlan>(p->np, p->npts_max, S(s));
     p->nmrows = d->spin ? 2 * d->lmax + 1 : d->lmax + 1;
@@ -266,6 +268,28 @@ int nusht_lenmap2gclm(nusht_plan_t p, nusht_ringplan_t ptg, const double* lenmap
     p->nu->set_pts(th, ph, npix, S(s));
     p->nu->type1(lenmap0, lenmap1, rot ? ga : nullptr, p->G.as<double2>(), p->np.ntheta, S(s));
     legendre_adjoint(p->tab, p->ncomp, p->G.as<double2>(), p->np.ntheta, 1, reinterpret_cast<double2*>(gclm), S(s));
+  });
+}
+
+// ------------------------------------------------------------------ stage timing
+int nusht_timing_enable(int on) {
+  return guard([&] { StageTimer::get().enable(on != 0); });
+}
+int nusht_timing_reset(void) {
+  return guard([&] { StageTimer::get().reset(); });
+}
+int nusht_timing_count(void) {
+  int n = -1;
+  int rc = guard([&] { n = StageTimer::get().collect(); });
+  return rc == NUSHT_OK ? n : -rc;
+}
+int nusht_timing_get(int i, const char** name, double* total_ms, int64_t* calls) {
+  return guard([&] {
+    auto& t = StageTimer::get();
+    NUSHT_REQUIRE(i >= 0 && i < t.count(), kErrArg, "timing index out of range");
+    if (name) *name = t.name(i);
+    if (total_ms) *total_ms = t.total_ms(i);
+    if (calls) *calls = t.calls(i);
   });
 }
 

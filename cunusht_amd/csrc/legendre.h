@@ -32,6 +32,9 @@ struct LegendreTables {
   DevBuf cnorm;           // double[nlm]     c_l
   DevBuf seeds;           // double4[(lmax+1) * nslot]  (mu_u, mu_v, k_u, k_v) at l0 = max(m, s)
   long nlm = 0;           // sum over m of (lmax - l0(m) + 1)
+  std::string tag = "rings";  // stage-timer label ("cc" for the nuSHT grid, "ring" for the pointing grid)
+  std::string n_prep, n_synth, n_adj;
+  int chunk = 2;          // maps per synthesis launch (register budget: 2 maps keep 4 waves per SIMD)
 
   void build(int lmax, int spin, const std::vector<double>& ring_theta, hipStream_t st);
   size_t bytes() const {
@@ -41,6 +44,7 @@ struct LegendreTables {
 };
 
 // Synthesis: alm -> G.  alm: nmaps x ncomp x nalm complex (healpy layout, stride alm_stride between maps).
+// Maps are processed `chunk` at a time; one recursion per (m, ring pair) serves all maps of a chunk.
 // coef_ws: workspace of legendre_coef_bytes() bytes.  G row of m: spin 0 -> m, spin>0 -> lmax + m
 // (negative m -> lmax - |m|); element (row, ring) at G + row * g_stride_m + ring * g_stride_r (+ map * g_stride_map).
 // fl (optional, device double[lmax+1]) multiplies E by fl[l] before synthesis.

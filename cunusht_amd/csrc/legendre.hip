@@ -10,6 +10,8 @@
 //               waves with LDS atomics, then written once.
 #include "legendre.h"
 
+#include "timing.h"
+
 #include <algorithm>
 #include <cmath>
 
@@ -510,6 +512,9 @@ void LegendreTables::build(int L, int s, const std::vector<double>& theta, hipSt
   lmax = L;
   spin = s;
   nring = (int)theta.size();
+  n_prep = "legendre_prep_" + tag + "_s" + std::to_string(s);
+  n_synth = "legendre_synth_" + tag + "_s" + std::to_string(s);
+  n_adj = "legendre_adjoint_" + tag + "_s" + std::to_string(s);
   // pair rings symmetric about the equator
   std::vector<int> order(nring);
   for (int i = 0; i < nring; ++i) order[i] = i;
@@ -573,12 +578,28 @@ static void launch_synth(const SynthArgs& a, int nblocks, hipStream_t st) {
   NUSHT_LAUNCH_CHECK();
 }
 
+static void legendre_synth_chunk(const LegendreTables& t, int nmaps, int ncomp, const double2* alm,
+                                 long alm_stride_map, const double* fl, double2* coef_ws, double2* G, long sm, long sr,
+                                 long smap, hipStream_t st);
+
 void legendre_synth(const LegendreTables& t, int nmaps, int ncomp, const double2* alm, long alm_stride_map,
                     const double* fl, double2* coef_ws, double2* G, long sm, long sr, long smap, hipStream_t st) {
   NUSHT_REQUIRE(nmaps >= 1 && nmaps <= 8, kErrArg, "legendre_synth: nmaps must be in [1, 8]");
+  const int ch = std::max(1, std::min(t.chunk, 8));
+  for (int b = 0; b < nmaps; b += ch) {
+    const int nb = std::min(ch, nmaps - b);
+    legendre_synth_chunk(t, nb, ncomp, alm + b * alm_stride_map, alm_stride_map, fl, coef_ws, G + b * smap, sm, sr,
+                         smap, st);
+  }
+}
+
+static void legendre_synth_chunk(const LegendreTables& t, int nmaps, int ncomp, const double2* alm,
+                                 long alm_stride_map, const double* fl, double2* coef_ws, double2* G, long sm, long sr,
+                                 long smap, hipStream_t st) {
   const int L = t.lmax;
   const int nbpad = padded_maps(nmaps);
   {
+    StageScope sc(t.n_prep.c_str(), st);
     dim3 grid(ceil_div(L + 1, 256), L + 1);
     hipLaunchKernelGGL(k_prep, grid, dim3(256), 0, st, L, t.spin, nmaps, nbpad, ncomp, alm, alm_stride_map, fl,
                        t.mofs.as<long>(), t.cnorm.as<double>(), coef_ws);
@@ -596,6 +617,7 @@ void legendre_synth(const LegendreTables& t, int nmaps, int ncomp, const double2
   a.G = G; a.sm = sm; a.sr = sr; a.smap = smap;
   const int nblocks = a.nsb * (L + 1);
   const bool sp = t.spin > 0;
+  StageScope sc(t.n_synth.c_str(), st);
   switch (nbpad) {  // the kernel's NB must equal the coefficient stride written by k_prep
     case 1: sp ? launch_synth<true, 1>(a, nblocks, st) : launch_synth<false, 1>(a, nblocks, st); break;
     case 2: sp ? launch_synth<true, 2>(a, nblocks, st) : launch_synth<false, 2>(a, nblocks, st); break;
@@ -624,6 +646,7 @@ void legendre_adjoint(const LegendreTables& t, int ncomp, const double2* G, long
   const int na = t.spin ? 2 : 1;
   const size_t lds = sizeof(double2) * (size_t)(L + 1) * na;
   NUSHT_REQUIRE(lds <= 160 * 1024, kErrUnsupported, "legendre_adjoint: lmax too large for the LDS accumulator");
+  StageScope sc(t.n_adj.c_str(), st);
   if (t.spin) {
     NUSHT_HIP(hipFuncSetAttribute((const void*)k_adjoint<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     hipLaunchKernelGGL(k_adjoint<true>, dim3(L + 1), dim3(64 * nw), lds, st, a);

@@ -8,6 +8,8 @@
 // is evaluated in the working precision.
 #include "nufft.h"
 
+#include "timing.h"
+
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -546,6 +548,7 @@ size_t NufftPlan::bytes() const {
 void NufftPlan::set_pts(const double* theta, const double* phi, long npts, hipStream_t st) {
   NUSHT_REQUIRE(npts >= 0 && npts <= npts_max_, kErrArg, "set_pts: npts exceeds the plan's maximum");
   npts_ = npts;
+  StageScope sc("set_pts", st);
   const int ntiles = ntile_r_ * ntile_c_;
   // tile_cnt_[ntiles + 1] is an error counter
   hipLaunchKernelGGL(k_fill_zero<int>, dim3(ceil_div(ntiles + 2, 256)), dim3(256), 0, st, tile_cnt_.as<int>(),
@@ -607,12 +610,19 @@ void NufftPlan::type2_impl(const double2* G, long sm, const double* gamma, doubl
   ThetaArgs<T> ta{p.lmax, p.spin, p.ntheta, p.ntd, p.nf, p.nm, p.row_lo, p.nrow, nmrows, G, sm,
                   deconv_.as<double>(), ftd, fnf, T_.as<T2>()};
   const size_t lds = sizeof(T2) * (size_t)std::max(p.ntd, p.nf);
-  NUSHT_LAUNCH_SIZED(k_theta2, T, std::max(p.ntd, p.nf), nmrows, lds, ta, st);
+  {
+    StageScope sc("theta_pass_t2", st);
+    NUSHT_LAUNCH_SIZED(k_theta2, T, std::max(p.ntd, p.nf), nmrows, lds, ta, st);
+  }
   PhiArgs<T> pa{p.lmax, p.nf, p.nm, p.nrow, fnf, T_.as<T2>(), grid_.as<T2>()};
-  NUSHT_LAUNCH_SIZED(k_phi2, T, p.nf, p.nrow, sizeof(T2) * (size_t)p.nf, pa, st);
+  {
+    StageScope sc("phi_pass_t2", st);
+    NUSHT_LAUNCH_SIZED(k_phi2, T, p.nf, p.nrow, sizeof(T2) * (size_t)p.nf, pa, st);
+  }
   if (npts_ == 0) return;
   InterpArgs<T> ia{p.nf, p.w, p.row_lo, p.nrow, p.spin, (T)p.beta, npts_, pts_.as<double2>(), idx_.as<int>(),
                    grid_.as<T2>(), gamma, ore, oim};
+  StageScope sc("nufft_interp", st);
   hipLaunchKernelGGL(k_interp<T>, dim3(ceil_div(npts_, 256)), dim3(256), 0, st, ia);
   NUSHT_LAUNCH_CHECK();
 }
@@ -625,10 +635,14 @@ void NufftPlan::type1_impl(const double* ire, const double* iim, const double* g
   const FftDesc<T> ftd = desc_of<T>(ftd_f_, ftd_d_);
   const FftDesc<T> fnf = desc_of<T>(fnf_f_, fnf_d_);
   const long ngrid = 2L * p.nrow * p.nf;
+  {
+  StageScope sc0("grid_zero", st);
   hipLaunchKernelGGL(k_fill_zero<T>, dim3(std::min<long>(ceil_div(ngrid, 256), 65536)), dim3(256), 0, st,
                      grid_.as<T>(), ngrid);
   NUSHT_LAUNCH_CHECK();
+  }
   if (npts_ > 0) {
+    StageScope sc1("nufft_spread", st);
     SpreadArgs<T> sa{p.nf, p.w, p.row_lo, p.nrow, p.spin, tile_r_, tile_c_, ntile_c_, (T)p.beta, pts_.as<double2>(),
                      idx_.as<int>(), tile_start_.as<int>(), ire, iim, gamma, grid_.as<T2>()};
     const size_t lds = sizeof(T2) * (size_t)(tile_r_ + p.w) * (tile_c_ + p.w);
@@ -636,11 +650,15 @@ void NufftPlan::type1_impl(const double* ire, const double* iim, const double* g
     NUSHT_LAUNCH_CHECK();
   }
   PhiArgs<T> pa{p.lmax, p.nf, p.nm, p.nrow, fnf, T_.as<T2>(), grid_.as<T2>()};
-  NUSHT_LAUNCH_SIZED(k_phi1, T, p.nf, p.nrow, sizeof(T2) * (size_t)p.nf, pa, st);
+  {
+    StageScope sc("phi_pass_t1", st);
+    NUSHT_LAUNCH_SIZED(k_phi1, T, p.nf, p.nrow, sizeof(T2) * (size_t)p.nf, pa, st);
+  }
   const int nmrows = p.spin ? 2 * p.lmax + 1 : p.lmax + 1;
   ThetaArgs<T> ta{p.lmax, p.spin, p.ntheta, p.ntd, p.nf, p.nm, p.row_lo, p.nrow, nmrows, G, sm,
                   deconv_.as<double>(), ftd, fnf, T_.as<T2>()};
   const size_t lds = sizeof(T2) * (size_t)std::max(p.ntd, p.nf);
+  StageScope sc("theta_pass_t1", st);
   NUSHT_LAUNCH_SIZED(k_theta1, T, std::max(p.ntd, p.nf), nmrows, lds, ta, st);
 }
 

@@ -6,6 +6,7 @@
 #include "ring.h"
 
 #include "stages.h"
+#include "timing.h"
 
 #include <algorithm>
 #include <cmath>
@@ -285,6 +286,7 @@ RingPlan::RingPlan(int lmax, int spin, const std::vector<double>& theta, const s
   upload(theta_, theta);
   upload(phi0_, phi0);
   upload(ofs_, ofs);
+  tab_.tag = "ring";
   tab_.build(lmax, spin, theta, st);
   nrows_ = spin ? 2 * lmax + 1 : lmax + 1;
   G_.alloc(sizeof(double2) * (size_t)nring_ * nrows_);
@@ -311,6 +313,8 @@ void RingPlan::ring_pass(int mode, double* o0, double* o1, double* o2, hipStream
   a.o0 = o0; a.o1 = o1; a.o2 = o2;
   a.i0 = o0; a.i1 = o1;
   const size_t lds = sizeof(double2) * (size_t)std::max(nphi_, mlen_);
+  static const char* names[4] = {"ring_dft_synth_s0", "ring_dft_synth_spin", "ring_dft_pointing", "ring_dft_adjoint"};
+  StageScope sc(names[mode], st);
   switch (method_) {
     case RingMethod::Direct: hipLaunchKernelGGL(k_ring<0>, dim3(nring_), dim3(kRingNT), lds, st, a); break;
     case RingMethod::Rader: hipLaunchKernelGGL(k_ring<1>, dim3(nring_), dim3(kRingNT), lds, st, a); break;

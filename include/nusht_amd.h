@@ -149,6 +149,15 @@ int nusht_lenmap2gclm(nusht_plan_t p, nusht_ringplan_t ptg, const double* lenmap
 int nusht_nusht2d2_batch(nusht_plan_t p, int nmaps, const double* alm, int64_t alm_stride, double* out0,
                          double* out1, int64_t out_stride, const double* gamma, nusht_stream_t s);
 
+/* ---------------------------------------------------------------- stage timing (no counterpart in the
+ * reference, whose timer decorators synchronize the device around every stage: helper_GPU.py:10-51).
+ * When enabled, instrumented launches are bracketed by hipEvents recorded on their own stream;
+ * nusht_timing_count() synchronizes on those events and returns the number of stage names. */
+int nusht_timing_enable(int on);
+int nusht_timing_reset(void);
+int nusht_timing_count(void);
+int nusht_timing_get(int i, const char** name, double* total_ms, int64_t* calls);
+
 #ifdef __cplusplus
 }
 #endif

@@ -250,3 +250,32 @@ def test_gclm2lenmap_vs_oracle(spin):
     ad = T.lenmap2gclm(_t(x), _t(dlm), None, L, L, spin=spin).cpu().numpy().reshape(nc, -1)
     refad = pipeline.lenmap2gclm(x, dlm, L, spin, nc, rings, 1e-13)
     assert _relmax(ad, refad) < 1e-9
+
+
+# ------------------------------------------------------------------ batched synthesis
+@pytest.mark.parametrize("nmaps", [1, 3, 8])
+def test_batch_matches_single_calls(nmaps):
+    """nusht_nusht2d2_batch (one Legendre recursion for all maps) == independent nusht2d2 calls."""
+    import cunusht_amd
+    from cunusht_amd.transformer import NuPlan
+    rng = np.random.default_rng(300 + nmaps)
+    L = 36
+    npts = 2000
+    loc = np.stack([np.arccos(rng.uniform(-1, 1, npts)), rng.uniform(0, 2 * np.pi, npts)], 1)
+    th, ph = _t(loc[:, 0]), _t(loc[:, 1])
+    gam = _t(rng.uniform(-0.1, 0.1, npts))
+    a = np.stack([almutil.random_alm(L, 2, 2, rng) for _ in range(nmaps)])
+    at = _t(a)
+    plan = NuPlan(L, 2, 2, 1e-10, npts)
+    plan.set_pts(th, ph)
+    out = torch.zeros((nmaps, 2, npts), dtype=torch.float64, device="cuda")
+    nalm = a.shape[2]
+    _lib().call("nusht_nusht2d2_batch", plan._h, nmaps, _p(at), 4 * nalm, _p(out[0, 0]), _p(out[0, 1]), 2 * npts,
+                _p(gam), _stream())
+    _sync()
+    for b in range(nmaps):
+        ref = torch.zeros((2, npts), dtype=torch.float64, device="cuda")
+        ab = _t(a[b])
+        _lib().call("nusht_nusht2d2", plan._h, _p(ab), _p(ref[0]), _p(ref[1]), _p(gam), _stream())
+        _sync()
+        assert torch.equal(out[b], ref) or _relmax(out[b].cpu().numpy(), ref.cpu().numpy()) < 1e-13
