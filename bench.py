@@ -266,7 +266,8 @@ def main():
                                f"batched {B} maps per shared deflection",
                    "lmax": L, "epsilon": eps, "batch": B, "npix": npix,
                    "nufft": {"nfine": info["nfine"], "nrow": info["nrow_fine"], "w": info["kernel_width"],
-                             "sigma": round(info["sigma"], 4)}},
+                             "sigma": round(info["sigma"], 4)},
+                   "legendre_variant": info["legendre_variant"]},
         "plain": {"value": rate_plain, "p50_ms": float(np.percentile(lat, 50)),
                   "p99_ms": float(np.percentile(lat, 99))},
         "served_over_plain": rate_batch / rate_plain,

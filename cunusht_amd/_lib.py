@@ -40,7 +40,8 @@ class PlanDesc(ctypes.Structure):
 
 class PlanInfo(ctypes.Structure):
     _fields_ = [("ntheta", _i), ("ntheta_dfs", _i), ("nfine", _i), ("nrow_fine", _i), ("kernel_width", _i),
-                ("single_precision", _i), ("sigma", _d), ("beta", _d), ("workspace_bytes", _i64)]
+                ("single_precision", _i), ("sigma", _d), ("beta", _d), ("workspace_bytes", _i64),
+                ("legendre_variant", _i), ("legendre_ms", ctypes.c_float)]
 
 
 # name -> argtypes (all return int unless listed in _RESTYPES)

@@ -160,6 +160,7 @@ int nusht_plan_create(const nusht_plan_desc* d, nusht_stream_t s, nusht_plan_t* 
     p->coef.alloc(legendre_coef_bytes(p->tab, nusht_plan_s::kMaxMaps));
     p->G.alloc(sizeof(double2) * (size_t)p->nmrows * p->np.ntheta * nusht_plan_s::kMaxMaps);
     p->ptg.alloc(sizeof(double) * 3 * (size_t)p->npts_max);
+    legendre_autotune(p->tab, p->ncomp, p->coef.as<double2>(), p->G.as<double2>(), p->np.ntheta, 1, S(s));
     NUSHT_HIP(hipStreamSynchronize(S(s)));
     *out = p.release();
   });
@@ -182,6 +183,8 @@ int nusht_plan_get_info(nusht_plan_t p, nusht_plan_info* info) {
     info->sigma = p->np.sigma;
     info->beta = p->np.beta;
     info->workspace_bytes = (int64_t)(p->tab.bytes() + p->nu->bytes() + p->coef.bytes() + p->G.bytes() + p->ptg.bytes());
+    info->legendre_variant = p->tab.variant;
+    info->legendre_ms = p->tab.tuned_ms;
   });
 }
 

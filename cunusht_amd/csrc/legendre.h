@@ -34,7 +34,9 @@ struct LegendreTables {
   long nlm = 0;           // sum over m of (lmax - l0(m) + 1)
   std::string tag = "rings";  // stage-timer label ("cc" for the nuSHT grid, "ring" for the pointing grid)
   std::string n_prep, n_synth, n_adj;
-  int chunk = 2;          // maps per synthesis launch (register budget: 2 maps keep 4 waves per SIMD)
+  int chunk = 2;          // maps per synthesis launch
+  int variant = -1;       // synthesis kernel variant (-1: default); set by legendre_autotune
+  float tuned_ms = 0.f;
 
   void build(int lmax, int spin, const std::vector<double>& ring_theta, hipStream_t st);
   size_t bytes() const {
@@ -52,6 +54,10 @@ size_t legendre_coef_bytes(const LegendreTables& t, int nmaps);
 void legendre_synth(const LegendreTables& t, int nmaps, int ncomp, const double2* alm, long alm_stride_map,
                     const double* fl, double2* coef_ws, double2* G, long g_stride_m, long g_stride_r,
                     long g_stride_map, hipStream_t st);
+
+// Times every synthesis variant on the given buffers (contents are overwritten) and keeps the fastest.
+// (Plan creation only: allocates a scratch alm set and synchronizes.)
+void legendre_autotune(LegendreTables& t, int ncomp, double2* coef_ws, double2* G, long sm, long sr, hipStream_t st);
 
 // Adjoint: G -> alm (exact transpose of legendre_synth under the real-field alm inner product).
 // For spin 0 only rows m = 0..lmax are read (the field is real); m = 0 imaginary parts are dropped.

@@ -13,6 +13,10 @@
 #include "timing.h"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <type_traits>
 #include <cmath>
 
 namespace nusht {
@@ -154,135 +158,143 @@ struct SynthArgs {
   long sm, sr, smap;
 };
 
-template <bool SP, int NB>
-__global__ __launch_bounds__(64) void k_synth(SynthArgs a) {
+// R ring pairs per lane (independent recursion chains); the l-loop runs in chunks of CH so
+// that the wave-uniform coefficients of a chunk are fetched together (one scalar-cache round
+// trip per chunk instead of per l).  Tables are padded by CH entries so a chunk never reads
+// past the end.
+constexpr int kSynCH = 8;  // largest chunk of any variant (table padding)
+
+template <bool SP, int NB, int R, int CH, int WPE>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void k_synth(SynthArgs a) {
+  constexpr int K = SP ? 2 * NB : NB;
   const int blk = blockIdx.x;
   const int m = blk / a.nsb;
   const int sb = blk - m * a.nsb;
-  const int slot = sb * 64 + threadIdx.x;
-  const bool active = slot < a.nslot;
   const int L = a.lmax;
   const int l0 = l0_of(m, a.s);
   if (l0 > L) return;
+  const int lane = threadIdx.x;
 
-  double x = 0.0, mu_u = 0.0, mu_v = 0.0, ku = 0.0, kv = 0.0;
-  if (active) {
-    x = cos(a.slot_theta[slot]);
-    double4 sd = a.seeds[(long)m * a.nslot + slot];
-    mu_u = sd.x; mu_v = sd.y; ku = sd.z; kv = sd.w;
+  double x[R], mu_u[R], mu_v[R], pu[R], pv[R], ku[R], kv[R];
+  int slot[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    slot[r] = sb * 64 * R + r * 64 + lane;
+    x[r] = mu_u[r] = mu_v[r] = pu[r] = pv[r] = ku[r] = kv[r] = 0.0;
+    if (slot[r] < a.nslot) {
+      x[r] = cos(a.slot_theta[slot[r]]);
+      double4 sd = a.seeds[(long)m * a.nslot + slot[r]];
+      mu_u[r] = sd.x; ku[r] = sd.z;
+      if (SP) { mu_v[r] = sd.y; kv[r] = sd.w; }
+    }
   }
-  if (!SP) { mu_v = 0.0; kv = 0.0; }
-  double pu = 0.0, pv = 0.0;
 
   const long base = a.mofs[m];
   const double2* __restrict__ rec = a.rec + base;
-  constexpr int K = SP ? 2 * NB : NB;
   const double2* __restrict__ cf = a.coef + base * K;
 
-  // accumulators: spin 0 -> acc[parity][b]; spin s -> N (no sign) and S (parity split) for +m and -m
-  double2 A0[NB], A1[NB];          // spin 0: parity-split sums; spin s: +m south parity sums
-  double2 PN[NB], MN[NB], MS0[NB], MS1[NB];
+  // spin 0: A0/A1 = parity-split sums; spin s: PN (+m north), MN (-m north), A (+m south), MS (-m south)
+  double2 A0[R][NB], A1[R][NB], PN[R][NB], MN[R][NB], MS0[R][NB], MS1[R][NB];
 #pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    A0[b] = A1[b] = double2{0.0, 0.0};
-    PN[b] = MN[b] = MS0[b] = MS1[b] = double2{0.0, 0.0};
-  }
-
-  int l = l0;
-  int rel = 0;  // (l - l0) parity toggle
-
-  auto contrib = [&](int j, double wu, double wv, bool odd) {
+  for (int r = 0; r < R; ++r)
 #pragma unroll
     for (int b = 0; b < NB; ++b) {
-      if (!SP) {
-        double2 c = cf[(long)j * NB + b];
-        if (odd) fma2(A1[b], c, wu); else fma2(A0[b], c, wu);
-      } else {
-        double2 w = cf[((long)j * NB + b) * 2];
-        double2 wt = cf[((long)j * NB + b) * 2 + 1];
-        fma2(PN[b], w, wu);
-        fma2(MN[b], wt, wv);
-        if (odd) { fma2(A1[b], w, wv); fma2(MS1[b], wt, wu); }
-        else { fma2(A0[b], w, wv); fma2(MS0[b], wt, wu); }
+      A0[r][b] = A1[r][b] = PN[r][b] = MN[r][b] = MS0[r][b] = MS1[r][b] = double2{0.0, 0.0};
+    }
+
+  // One chunk of CH consecutive l (the first at even l - l0).  SLOW: some lane may still be
+  // below the representable range (masked contributions, rescaling); FULL: all CH l <= L.
+  auto chunk = [&](int lc, auto slow_tag, auto full_tag) {
+    constexpr bool SLOW = decltype(slow_tag)::value;
+    constexpr bool FULL = decltype(full_tag)::value;
+    const int j0 = lc - l0;
+    double2 rc[CH];
+    double2 cw[CH][K];
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      rc[i] = rec[j0 + i];
+#pragma unroll
+      for (int k = 0; k < K; ++k) cw[i][k] = cf[(long)(j0 + i) * K + k];
+    }
+#pragma unroll
+    for (int i = 0; i < CH; ++i) {
+      if (!FULL && lc + i > L) break;
+      const bool odd = (i & 1) != 0;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        double wu = mu_u[r], wv = mu_v[r];
+        if (SLOW) { wu = ku[r] == 0.0 ? wu : 0.0; wv = kv[r] == 0.0 ? wv : 0.0; }
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+          if (!SP) {
+            if (odd) fma2(A1[r][b], cw[i][b], wu); else fma2(A0[r][b], cw[i][b], wu);
+          } else {
+            const double2 w = cw[i][2 * b], wt = cw[i][2 * b + 1];
+            fma2(PN[r][b], w, wu);
+            fma2(MN[r][b], wt, wv);
+            if (odd) { fma2(A1[r][b], w, wv); fma2(MS1[r][b], wt, wu); }
+            else { fma2(A0[r][b], w, wv); fma2(MS0[r][b], wt, wu); }
+          }
+        }
+      }
+      // the step past l = lmax reads a padded table entry; its result is never used
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const double t = rc[i].x * x[r];
+        const double nu = fma(t + rc[i].y, mu_u[r], -pu[r]);
+        pu[r] = mu_u[r]; mu_u[r] = nu;
+        if (SP) {
+          const double nv = fma(t - rc[i].y, mu_v[r], -pv[r]);
+          pv[r] = mu_v[r]; mu_v[r] = nv;
+        }
+        if (SLOW) {
+          if (ku[r] < 0.0 && fabs(mu_u[r]) > kThresh) { mu_u[r] *= kSmall; pu[r] *= kSmall; ku[r] += 1.0; }
+          if (SP && kv[r] < 0.0 && fabs(mu_v[r]) > kThresh) { mu_v[r] *= kSmall; pv[r] *= kSmall; kv[r] += 1.0; }
+        }
       }
     }
   };
+  using T_ = std::true_type;
+  using F_ = std::false_type;
+  int lc = l0;
+  // slow phase: while some lane of the wave is still below 2^-300
+  for (; lc <= L; lc += CH) {
+    bool under = false;
+#pragma unroll
+    for (int r = 0; r < R; ++r) under |= (ku[r] < 0.0) || (kv[r] < 0.0);
+    if (!__any(under)) break;
+    if (lc + CH - 1 <= L) chunk(lc, T_{}, T_{});
+    else chunk(lc, T_{}, F_{});
+  }
+  // fast phase
+  for (; lc + CH - 1 <= L; lc += CH) chunk(lc, F_{}, T_{});
+  if (lc <= L) chunk(lc, F_{}, F_{});
 
-  // phase 1: some lane still underflowed -> masked contributions and rescaling
-  while (l <= L && __any((ku < 0.0) || (kv < 0.0))) {
-    const int j = l - l0;
-    contrib(j, ku == 0.0 ? mu_u : 0.0, kv == 0.0 ? mu_v : 0.0, rel);
-    if (l < L) {
-      double2 r = rec[j];
-      double t = r.x * x;
-      double nu = fma(t + r.y, mu_u, -pu);
-      pu = mu_u; mu_u = nu;
-      if (ku < 0.0 && fabs(mu_u) > kThresh) { mu_u *= kSmall; pu *= kSmall; ku += 1.0; }
-      if (SP) {
-        double nv = fma(t - r.y, mu_v, -pv);
-        pv = mu_v; mu_v = nv;
-        if (kv < 0.0 && fabs(mu_v) > kThresh) { mu_v *= kSmall; pv *= kSmall; kv += 1.0; }
-      }
-    }
-    ++l;
-    rel ^= 1;
-  }
-  // align to an even (l - l0) so that the unrolled pair is (even, odd)
-  if (rel && l <= L) {
-    const int j = l - l0;
-    contrib(j, mu_u, mu_v, true);
-    if (l < L) {
-      double2 r = rec[j];
-      double t = r.x * x;
-      double nu = fma(t + r.y, mu_u, -pu);
-      pu = mu_u; mu_u = nu;
-      if (SP) { double nv = fma(t - r.y, mu_v, -pv); pv = mu_v; mu_v = nv; }
-    }
-    ++l;
-    rel = 0;
-  }
-  // phase 2: plain recursion, unrolled by two
-  for (; l + 1 <= L; l += 2) {
-    const int j = l - l0;
-    double2 r0 = rec[j];
-    double2 r1 = rec[j + 1];
-    contrib(j, mu_u, mu_v, false);
-    double t = r0.x * x;
-    double nu = fma(t + r0.y, mu_u, -pu);
-    pu = mu_u; mu_u = nu;
-    if (SP) { double nv = fma(t - r0.y, mu_v, -pv); pv = mu_v; mu_v = nv; }
-    contrib(j + 1, mu_u, mu_v, true);
-    if (l + 1 < L) {
-      t = r1.x * x;
-      nu = fma(t + r1.y, mu_u, -pu);
-      pu = mu_u; mu_u = nu;
-      if (SP) { double nv = fma(t - r1.y, mu_v, -pv); pv = mu_v; mu_v = nv; }
-    }
-  }
-  if (l <= L) contrib(l - l0, mu_u, mu_v, false);
-
-  if (!active) return;
-  const int rn = a.slot_north[slot];
-  const int rs = a.slot_south[slot];
   const double sg0 = ((l0 + m) & 1) ? -1.0 : 1.0;  // (-1)^{l+m} for even (l - l0)
 #pragma unroll
-  for (int b = 0; b < NB; ++b) {
-    if (b >= a.nmaps) break;
-    double2* Gb = a.G + b * a.smap;
-    if (!SP) {
-      double2 gn = cadd(A0[b], A1[b]);
-      double2 gs = double2{sg0 * (A0[b].x - A1[b].x), sg0 * (A0[b].y - A1[b].y)};
-      Gb[(long)m * a.sm + (long)rn * a.sr] = gn;
-      if (rs >= 0) Gb[(long)m * a.sm + (long)rs * a.sr] = gs;
-    } else {
-      double2 pS = double2{sg0 * (A0[b].x - A1[b].x), sg0 * (A0[b].y - A1[b].y)};
-      double2 mS = double2{sg0 * (MS0[b].x - MS1[b].x), sg0 * (MS0[b].y - MS1[b].y)};
-      const long rowp = (long)(L + m) * a.sm, rowm = (long)(L - m) * a.sm;
-      Gb[rowp + (long)rn * a.sr] = PN[b];
-      if (rs >= 0) Gb[rowp + (long)rs * a.sr] = pS;
-      if (m > 0) {
-        Gb[rowm + (long)rn * a.sr] = MN[b];
-        if (rs >= 0) Gb[rowm + (long)rs * a.sr] = mS;
+  for (int r = 0; r < R; ++r) {
+    if (slot[r] >= a.nslot) continue;
+    const int rn = a.slot_north[slot[r]];
+    const int rs = a.slot_south[slot[r]];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      if (b >= a.nmaps) break;
+      double2* Gb = a.G + b * a.smap;
+      if (!SP) {
+        double2 gn = cadd(A0[r][b], A1[r][b]);
+        double2 gs = double2{sg0 * (A0[r][b].x - A1[r][b].x), sg0 * (A0[r][b].y - A1[r][b].y)};
+        Gb[(long)m * a.sm + (long)rn * a.sr] = gn;
+        if (rs >= 0) Gb[(long)m * a.sm + (long)rs * a.sr] = gs;
+      } else {
+        double2 pS = double2{sg0 * (A0[r][b].x - A1[r][b].x), sg0 * (A0[r][b].y - A1[r][b].y)};
+        double2 mS = double2{sg0 * (MS0[r][b].x - MS1[r][b].x), sg0 * (MS0[r][b].y - MS1[r][b].y)};
+        const long rowp = (long)(L + m) * a.sm, rowm = (long)(L - m) * a.sm;
+        Gb[rowp + (long)rn * a.sr] = PN[r][b];
+        if (rs >= 0) Gb[rowp + (long)rs * a.sr] = pS;
+        if (m > 0) {
+          Gb[rowm + (long)rn * a.sr] = MN[r][b];
+          if (rs >= 0) Gb[rowm + (long)rs * a.sr] = mS;
+        }
       }
     }
   }
@@ -555,8 +567,9 @@ void LegendreTables::build(int L, int s, const std::vector<double>& theta, hipSt
   }
   nlm = mo[L + 1];
   upload(mofs, mo);
-  rec.alloc(sizeof(double2) * (size_t)std::max<long>(nlm, 1));
-  cnorm.alloc(sizeof(double) * (size_t)std::max<long>(nlm, 1));
+  // padded so that a chunked l-loop may read up to kSynCH entries past the last one
+  rec.alloc(sizeof(double2) * (size_t)(std::max<long>(nlm, 1) + kSynCH));
+  cnorm.alloc(sizeof(double) * (size_t)(std::max<long>(nlm, 1) + kSynCH));
   seeds.alloc(sizeof(double4) * (size_t)(L + 1) * nslot);
   hipLaunchKernelGGL(k_rec_tables, dim3(ceil_div(L + 1, 128)), dim3(128), 0, st, L, s, mofs.as<long>(),
                      rec.as<double2>(), cnorm.as<double>());
@@ -569,12 +582,31 @@ void LegendreTables::build(int L, int s, const std::vector<double>& theta, hipSt
 static int padded_maps(int nmaps) { return nmaps <= 1 ? 1 : nmaps <= 2 ? 2 : nmaps <= 4 ? 4 : 8; }
 
 size_t legendre_coef_bytes(const LegendreTables& t, int nmaps) {
-  return sizeof(double2) * (size_t)std::max<long>(t.nlm, 1) * padded_maps(nmaps) * (t.spin ? 2 : 1);
+  // padded by kSynCH entries (chunked reads past the last l of the last m)
+  return sizeof(double2) * (size_t)(std::max<long>(t.nlm, 1) + kSynCH) * padded_maps(nmaps) * (t.spin ? 2 : 1);
 }
 
+// Synthesis variants (ring pairs per lane R, l-chunk CH, minimum waves per SIMD); the
+// autotuner picks one per table.
+struct SynVariant {
+  int R, CH, WPE;
+};
+static const SynVariant kSynVariants[] = {{1, 4, 1}, {1, 8, 1}, {2, 2, 1}, {2, 4, 1}, {1, 2, 1}};
+constexpr int kNumSynVariants = 5;
+
 template <bool SP, int NB>
-static void launch_synth(const SynthArgs& a, int nblocks, hipStream_t st) {
-  hipLaunchKernelGGL((k_synth<SP, NB>), dim3(nblocks), dim3(64), 0, st, a);
+static void launch_synth(const SynthArgs& a, int variant, hipStream_t st) {
+  const SynVariant v = kSynVariants[variant];
+  const int nblocks = ceil_div(a.nslot, 64 * v.R) * (a.lmax + 1);
+  SynthArgs b = a;
+  b.nsb = ceil_div(a.nslot, 64 * v.R);
+  switch (variant) {
+    case 0: hipLaunchKernelGGL((k_synth<SP, NB, 1, 4, 1>), dim3(nblocks), dim3(64), 0, st, b); break;
+    case 1: hipLaunchKernelGGL((k_synth<SP, NB, 1, 8, 1>), dim3(nblocks), dim3(64), 0, st, b); break;
+    case 2: hipLaunchKernelGGL((k_synth<SP, NB, 2, 2, 1>), dim3(nblocks), dim3(64), 0, st, b); break;
+    case 3: hipLaunchKernelGGL((k_synth<SP, NB, 2, 4, 1>), dim3(nblocks), dim3(64), 0, st, b); break;
+    default: hipLaunchKernelGGL((k_synth<SP, NB, 1, 2, 1>), dim3(nblocks), dim3(64), 0, st, b); break;
+  }
   NUSHT_LAUNCH_CHECK();
 }
 
@@ -585,7 +617,7 @@ static void legendre_synth_chunk(const LegendreTables& t, int nmaps, int ncomp, 
 void legendre_synth(const LegendreTables& t, int nmaps, int ncomp, const double2* alm, long alm_stride_map,
                     const double* fl, double2* coef_ws, double2* G, long sm, long sr, long smap, hipStream_t st) {
   NUSHT_REQUIRE(nmaps >= 1 && nmaps <= 8, kErrArg, "legendre_synth: nmaps must be in [1, 8]");
-  const int ch = std::max(1, std::min(t.chunk, 8));
+  const int ch = std::max(1, std::min(t.chunk, 2));
   for (int b = 0; b < nmaps; b += ch) {
     const int nb = std::min(ch, nmaps - b);
     legendre_synth_chunk(t, nb, ncomp, alm + b * alm_stride_map, alm_stride_map, fl, coef_ws, G + b * smap, sm, sr,
@@ -606,7 +638,7 @@ static void legendre_synth_chunk(const LegendreTables& t, int nmaps, int ncomp, 
     NUSHT_LAUNCH_CHECK();
   }
   SynthArgs a;
-  a.lmax = L; a.s = t.spin; a.nslot = t.nslot; a.nsb = ceil_div(t.nslot, 64); a.nmaps = nmaps;
+  a.lmax = L; a.s = t.spin; a.nslot = t.nslot; a.nsb = 0; a.nmaps = nmaps;
   a.slot_theta = t.slot_theta.as<double>();
   a.slot_north = t.slot_north.as<int>();
   a.slot_south = t.slot_south.as<int>();
@@ -615,15 +647,51 @@ static void legendre_synth_chunk(const LegendreTables& t, int nmaps, int ncomp, 
   a.seeds = t.seeds.as<double4>();
   a.coef = coef_ws;
   a.G = G; a.sm = sm; a.sr = sr; a.smap = smap;
-  const int nblocks = a.nsb * (L + 1);
   const bool sp = t.spin > 0;
+  const int var = (t.variant >= 0 && t.variant < kNumSynVariants) ? t.variant : 3;
   StageScope sc(t.n_synth.c_str(), st);
   switch (nbpad) {  // the kernel's NB must equal the coefficient stride written by k_prep
-    case 1: sp ? launch_synth<true, 1>(a, nblocks, st) : launch_synth<false, 1>(a, nblocks, st); break;
-    case 2: sp ? launch_synth<true, 2>(a, nblocks, st) : launch_synth<false, 2>(a, nblocks, st); break;
-    case 4: sp ? launch_synth<true, 4>(a, nblocks, st) : launch_synth<false, 4>(a, nblocks, st); break;
-    default: sp ? launch_synth<true, 8>(a, nblocks, st) : launch_synth<false, 8>(a, nblocks, st); break;
+    case 1: sp ? launch_synth<true, 1>(a, var, st) : launch_synth<false, 1>(a, var, st); break;
+    case 2: sp ? launch_synth<true, 2>(a, var, st) : launch_synth<false, 2>(a, var, st); break;
+    default: throw Error{kErrUnsupported, "legendre_synth: at most 2 maps per launch"};
   }
+}
+
+void legendre_autotune(LegendreTables& t, int ncomp, double2* coef_ws, double2* G, long sm, long sr, hipStream_t st) {
+  // one measurement per (lmax, spin, ring pairs) and process
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int, int>, std::pair<int, float>> cache;
+  const auto key = std::make_tuple(t.lmax, t.spin, t.nslot, ncomp);
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) { t.variant = it->second.first; t.tuned_ms = it->second.second; return; }
+  }
+  DevBuf dummy(sizeof(double2) * (size_t)(t.lmax + 1) * (t.lmax + 2) / 2 * ncomp);
+  NUSHT_HIP(hipMemsetAsync(dummy.get(), 0, dummy.bytes(), st));
+  const double2* alm = dummy.as<double2>();
+  hipEvent_t e0, e1;
+  NUSHT_HIP(hipEventCreate(&e0));
+  NUSHT_HIP(hipEventCreate(&e1));
+  float best = 1e30f;
+  int bestv = 3;
+  for (int v = 0; v < kNumSynVariants; ++v) {
+    t.variant = v;
+    legendre_synth(t, 1, ncomp, alm, 0, nullptr, coef_ws, G, sm, sr, 0, st);  // warm-up
+    NUSHT_HIP(hipEventRecord(e0, st));
+    for (int k = 0; k < 2; ++k) legendre_synth(t, 1, ncomp, alm, 0, nullptr, coef_ws, G, sm, sr, 0, st);
+    NUSHT_HIP(hipEventRecord(e1, st));
+    NUSHT_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    NUSHT_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (ms < best) { best = ms; bestv = v; }
+  }
+  t.variant = bestv;
+  t.tuned_ms = best / 2;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  std::lock_guard<std::mutex> g(mu);
+  cache[key] = {t.variant, t.tuned_ms};
 }
 
 void legendre_adjoint(const LegendreTables& t, int ncomp, const double2* G, long sm, long sr, double2* alm,

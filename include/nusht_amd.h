@@ -118,6 +118,8 @@ typedef struct {
   double sigma;
   double beta;
   int64_t workspace_bytes;
+  int legendre_variant;  /* synthesis kernel variant chosen by the plan-time autotuner */
+  float legendre_ms;     /* its measured time for one map */
 } nusht_plan_info;
 
 int nusht_plan_create(const nusht_plan_desc* desc, nusht_stream_t s, nusht_plan_t* out);
