@@ -23,6 +23,7 @@ struct LegendreTables {
   int lmax = 0;
   int spin = 0;
   int nslot = 0;          // ring pairs + unpaired rings
+  int sstride = 0;        // row stride of the seed table (odd, >= nslot)
   int nring = 0;
   DevBuf slot_theta;      // double[nslot]   theta of the slot's northern ring
   DevBuf slot_north;      // int[nslot]      ring index

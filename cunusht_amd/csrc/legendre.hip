@@ -63,8 +63,8 @@ __device__ inline void renorm(double& v, double& k) {
 }
 
 // Starting values at l0 = max(m, s): u = N_l0 d^l0_{m,-s}, v = N_l0 d^l0_{m,+s}, with scale exponents.
-__global__ __launch_bounds__(64) void k_seeds(int lmax, int s, int nslot, const double* __restrict__ slot_theta,
-                        double4* __restrict__ seeds) {
+__global__ __launch_bounds__(64) void k_seeds(int lmax, int s, int nslot, int sstride,
+                                              const double* __restrict__ slot_theta, double4* __restrict__ seeds) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nslot) return;
   const double th = slot_theta[i];
@@ -78,7 +78,7 @@ __global__ __launch_bounds__(64) void k_seeds(int lmax, int s, int nslot, const 
     double uu = ns * bin * pow(ch, (double)(s - m)) * pow(sh, (double)(s + m));
     if ((m + s) & 1) uu = -uu;
     double vv = ns * bin * pow(ch, (double)(s + m)) * pow(sh, (double)(s - m));
-    seeds[(long)m * nslot + i] = double4{uu, vv, 0.0, 0.0};
+    seeds[(long)m * sstride + i] = double4{uu, vv, 0.0, 0.0};
   }
   if (s > lmax) return;
   // m = s: u = N_s sin^{2s}(th/2), v = N_s cos^{2s}(th/2); then the product recursion in m.
@@ -86,14 +86,14 @@ __global__ __launch_bounds__(64) void k_seeds(int lmax, int s, int nslot, const 
   for (int j = 0; j < 2 * s; ++j) { u *= sh; v *= ch; }
   renorm(u, ku);
   renorm(v, kv);
-  seeds[(long)s * nslot + i] = double4{u, v, ku, kv};
+  seeds[(long)s * sstride + i] = double4{u, v, ku, kv};
   for (int m = s + 1; m <= lmax; ++m) {
     double f = -sqrt(2.0 * m * (2.0 * m + 1) / ((double)m * m - (double)s * s)) * sh * ch;
     u *= f;
     v *= f;
     renorm(u, ku);
     renorm(v, kv);
-    seeds[(long)m * nslot + i] = double4{u, v, ku, kv};
+    seeds[(long)m * sstride + i] = double4{u, v, ku, kv};
   }
 }
 
@@ -146,7 +146,7 @@ __device__ __forceinline__ void fma2(double2& acc, double2 c, double w) {
 }
 
 struct SynthArgs {
-  int lmax, s, nslot, nsb, nmaps;
+  int lmax, s, nslot, sstride, nsb, nmaps;
   const double* slot_theta;
   const int* slot_north;
   const int* slot_south;
@@ -183,7 +183,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
     x[r] = mu_u[r] = mu_v[r] = pu[r] = pv[r] = ku[r] = kv[r] = 0.0;
     if (slot[r] < a.nslot) {
       x[r] = cos(a.slot_theta[slot[r]]);
-      double4 sd = a.seeds[(long)m * a.nslot + slot[r]];
+      double4 sd = a.seeds[(long)m * a.sstride + slot[r]];
       mu_u[r] = sd.x; ku[r] = sd.z;
       if (SP) { mu_v[r] = sd.y; kv[r] = sd.w; }
     }
@@ -356,7 +356,7 @@ __device__ __forceinline__ int reduce_index() {
 }
 
 struct AdjArgs {
-  int lmax, s, nslot, ncomp;
+  int lmax, s, nslot, sstride, ncomp;
   const double* slot_theta;
   const int* slot_north;
   const int* slot_south;
@@ -410,7 +410,7 @@ __global__ __launch_bounds__(AdjCfg<SP>::MaxThreads) void k_adjoint(AdjArgs a) {
     P0[r] = P1[r] = Q0[r] = Q1[r] = double2{0.0, 0.0};
     if (slot < a.nslot) {
       x[r] = cos(a.slot_theta[slot]);
-      double4 sd = a.seeds[(long)m * a.nslot + slot];
+      double4 sd = a.seeds[(long)m * a.sstride + slot];
       mu[r] = sd.x; ku[r] = sd.z;
       if (SP) { nv_[r] = sd.y; kv[r] = sd.w; }
       const int rn = a.slot_north[slot];
@@ -570,12 +570,16 @@ void LegendreTables::build(int L, int s, const std::vector<double>& theta, hipSt
   // padded so that a chunked l-loop may read up to kSynCH entries past the last one
   rec.alloc(sizeof(double2) * (size_t)(std::max<long>(nlm, 1) + kSynCH));
   cnorm.alloc(sizeof(double) * (size_t)(std::max<long>(nlm, 1) + kSynCH));
-  seeds.alloc(sizeof(double4) * (size_t)(L + 1) * nslot);
+  // seed rows are padded to an odd number of entries: a power-of-two row stride (2048 ring pairs
+  // = 64 KiB) measurably slows the synthesis
+  sstride = nslot | 1;
+  if (sstride == nslot) sstride += 2;
+  seeds.alloc(sizeof(double4) * (size_t)(L + 1) * sstride);
   hipLaunchKernelGGL(k_rec_tables, dim3(ceil_div(L + 1, 128)), dim3(128), 0, st, L, s, mofs.as<long>(),
                      rec.as<double2>(), cnorm.as<double>());
   NUSHT_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_seeds, dim3(ceil_div(nslot, 64)), dim3(64), 0, st, L, s, nslot, slot_theta.as<double>(),
-                     seeds.as<double4>());
+  hipLaunchKernelGGL(k_seeds, dim3(ceil_div(nslot, 64)), dim3(64), 0, st, L, s, nslot, sstride,
+                     slot_theta.as<double>(), seeds.as<double4>());
   NUSHT_LAUNCH_CHECK();
 }
 
@@ -638,7 +642,7 @@ static void legendre_synth_chunk(const LegendreTables& t, int nmaps, int ncomp, 
     NUSHT_LAUNCH_CHECK();
   }
   SynthArgs a;
-  a.lmax = L; a.s = t.spin; a.nslot = t.nslot; a.nsb = 0; a.nmaps = nmaps;
+  a.lmax = L; a.s = t.spin; a.nslot = t.nslot; a.sstride = t.sstride; a.nsb = 0; a.nmaps = nmaps;
   a.slot_theta = t.slot_theta.as<double>();
   a.slot_north = t.slot_north.as<int>();
   a.slot_south = t.slot_south.as<int>();
@@ -698,7 +702,7 @@ void legendre_adjoint(const LegendreTables& t, int ncomp, const double2* G, long
                       hipStream_t st) {
   const int L = t.lmax;
   AdjArgs a;
-  a.lmax = L; a.s = t.spin; a.nslot = t.nslot; a.ncomp = ncomp;
+  a.lmax = L; a.s = t.spin; a.nslot = t.nslot; a.sstride = t.sstride; a.ncomp = ncomp;
   a.slot_theta = t.slot_theta.as<double>();
   a.slot_north = t.slot_north.as<int>();
   a.slot_south = t.slot_south.as<int>();

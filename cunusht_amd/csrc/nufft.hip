@@ -230,74 +230,92 @@ __global__ __launch_bounds__(NT) void k_theta1(ThetaArgs<T> a) {
 // ------------------------------------------------------------------ interpolation (type 2)
 template <typename T>
 struct InterpArgs {
-  int nf, w, row_lo, nrow, spin;
+  int nf, w, row_lo, nrow, spin, tr, tc, ntc;
   T beta;
   long npts;
   const double2* pts;
   const int* idx;
+  const int* tile_start;
   const c2_t<T>* grid;
   const double* gamma;
   double* out_re;
   double* out_im;
 };
 
+// One workgroup per fine-grid tile: the tile plus its (w-1)-cell halo is staged in LDS once and
+// every point binned to the tile is interpolated from it.
 template <typename T>
 __global__ __launch_bounds__(256) void k_interp(InterpArgs<T> a) {
   using T2 = c2_t<T>;
-  const long p = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (p >= a.npts) return;
-  const double2 tp = a.pts[p];
+  extern __shared__ unsigned char smem_raw[];
+  T2* tile = reinterpret_cast<T2*>(smem_raw);
+  const int tb = blockIdx.x;
+  const int s0 = a.tile_start[tb], s1 = a.tile_start[tb + 1];
+  if (s0 == s1) return;
+  const int tw = a.tc + a.w, th = a.tr + a.w;
+  const int tile_r = tb / a.ntc, tile_c = tb - (tb / a.ntc) * a.ntc;
+  const int r0 = tile_r * a.tr, c0 = tile_c * a.tc;
+  for (int i = threadIdx.x; i < tw * th; i += blockDim.x) {
+    const int rr = i / tw, cc = i - rr * tw;
+    const int grow = r0 + rr;
+    int gc = c0 + cc;
+    while (gc >= a.nf) gc -= a.nf;
+    tile[i] = grow < a.nrow ? a.grid[(long)grow * a.nf + gc] : T2{T(0), T(0)};
+  }
+  __syncthreads();
   const double scale = a.nf / (2.0 * M_PI);
   const double hw = 0.5 * a.w;
-  const double ut = tp.x * scale;
-  double up = tp.y * scale;
-  up -= a.nf * floor(up / a.nf);
-  const int i0 = (int)ceil(ut - hw);
-  const int j0 = (int)ceil(up - hw);
-  const int q0 = i0 - a.row_lo;
   const T inv = (T)(2.0 / a.w);
-  T kt[kMaxW], kp[kMaxW];
-  const double ft = i0 - ut, fp = j0 - up;
+  for (int p = s0 + threadIdx.x; p < s1; p += blockDim.x) {
+    const double2 tp = a.pts[p];
+    const double ut = tp.x * scale;
+    double up = tp.y * scale;
+    up -= a.nf * floor(up / a.nf);
+    const int i0 = (int)ceil(ut - hw);
+    const int j0u = (int)ceil(up - hw);
+    const int j0 = j0u < 0 ? j0u + a.nf : j0u;
+    const int lr = i0 - a.row_lo - r0, lc = j0 - c0;
+    const int o = a.idx[p];
+    const bool ok = lr >= 0 && lr < a.tr && lc >= 0 && lc < a.tc;  // always true for binned points
+    T kt[kMaxW], kp[kMaxW];
+    const double ft = i0 - ut, fp = j0u - up;
 #pragma unroll
-  for (int i = 0; i < kMaxW; ++i) {
-    if (i < a.w) {
-      kt[i] = WorkT<T>::ker((T)(ft + i) * inv, a.beta);
-      kp[i] = WorkT<T>::ker((T)(fp + i) * inv, a.beta);
-    }
-  }
-  T sr = 0, si = 0;
-  const bool inband = q0 >= 0 && q0 + a.w <= a.nrow;
-  if (inband) {
-    int jc = j0 < 0 ? j0 + a.nf : j0;
-    for (int i = 0; i < a.w; ++i) {
-      const T2* row = a.grid + (long)(q0 + i) * a.nf;
-      T rr = 0, ri = 0;
-      int c = jc;
-      for (int j = 0; j < a.w; ++j) {
-        T2 g = row[c];
-        rr = fma(g.x, kp[j], rr);
-        ri = fma(g.y, kp[j], ri);
-        ++c;
-        if (c == a.nf) c = 0;
+    for (int i = 0; i < kMaxW; ++i) {
+      if (i < a.w) {
+        kt[i] = WorkT<T>::ker((T)(ft + i) * inv, a.beta);
+        kp[i] = WorkT<T>::ker((T)(fp + i) * inv, a.beta);
       }
-      sr = fma(rr, kt[i], sr);
-      si = fma(ri, kt[i], si);
     }
-  }
-  const int o = a.idx[p];
-  double vr = inband ? (double)sr : __builtin_nan("");
-  double vi = inband ? (double)si : __builtin_nan("");
-  if (a.spin == 0) {
-    a.out_re[o] = vr;
-  } else {
-    if (a.gamma) {
-      double s, c;
-      sincos(a.spin * a.gamma[o], &s, &c);
-      double xr = vr * c - vi * s, xi = vr * s + vi * c;
-      vr = xr; vi = xi;
+    T sr = 0, si = 0;
+    if (ok) {
+      for (int i = 0; i < a.w; ++i) {
+        const T2* row = tile + (lr + i) * tw + lc;
+        T rr = 0, ri = 0;
+#pragma unroll 4
+        for (int j = 0; j < a.w; ++j) {
+          const T2 g = row[j];
+          rr = fma(g.x, kp[j], rr);
+          ri = fma(g.y, kp[j], ri);
+        }
+        sr = fma(rr, kt[i], sr);
+        si = fma(ri, kt[i], si);
+      }
     }
-    a.out_re[o] = vr;
-    a.out_im[o] = vi;
+    double vr = ok ? (double)sr : __builtin_nan("");
+    double vi = ok ? (double)si : __builtin_nan("");
+    if (a.spin == 0) {
+      a.out_re[o] = vr;
+    } else {
+      if (a.gamma) {
+        double s, c;
+        sincos(a.spin * a.gamma[o], &s, &c);
+        const double xr = vr * c - vi * s, xi = vr * s + vi * c;
+        vr = xr;
+        vi = xi;
+      }
+      a.out_re[o] = vr;
+      a.out_im[o] = vi;
+    }
   }
 }
 
@@ -390,36 +408,76 @@ struct BinArgs {
   int* cnt;
 };
 
-__global__ void k_bin_count(BinArgs a, int* bad) {
-  const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (j >= a.npts) return;
+// Wave-aggregated binning: lanes holding consecutive points with the same tile form a run;
+// the run's first lane issues one atomic for the whole run, and the run keeps its input order
+// (ring-ordered lensing points mostly arrive tile by tile, so this is close to one atomic per wave).
+struct RunInfo {
+  int leader;  // lane index of this lane's run leader
+  int len;     // run length (valid on the leader)
+};
+
+__device__ __forceinline__ RunInfo wave_runs(int key, bool valid) {
+  const int lane = threadIdx.x & 63;
+  const int kprev = __shfl_up(key, 1, 64);
+  const int vprev = __shfl_up((int)valid, 1, 64);
+  const bool cont = valid && lane > 0 && vprev && kprev == key;
+  const bool lead = valid && !cont;
+  const unsigned long long cm = __ballot(cont);
+  const unsigned long long lm = __ballot(lead);
+  RunInfo r;
+  const unsigned long long upto = (lane == 63) ? ~0ull : ((2ull << lane) - 1ull);
+  const unsigned long long lmask = lm & upto;
+  r.leader = lmask ? 63 - __clzll((long long)lmask) : lane;
+  // length: 1 + number of consecutive continuation bits above the leader
+  unsigned long long above = (lane == 63) ? 0ull : (cm >> (lane + 1));
+  r.len = 1 + (int)__builtin_ctzll(~above);
+  if (r.len > 64 - lane) r.len = 64 - lane;
+  return r;
+}
+
+__device__ __forceinline__ int bin_key(const BinArgs& a, double theta, double phi, bool& bad) {
   const double scale = a.nf / (2.0 * M_PI);
   const double hw = 0.5 * a.w;
-  const double ut = a.theta[j] * scale;
-  double up = a.phi[j] * scale;
+  const double ut = theta * scale;
+  double up = phi * scale;
   up -= a.nf * floor(up / a.nf);
   const int q0 = (int)ceil(ut - hw) - a.row_lo;
   int j0 = (int)ceil(up - hw);
   if (j0 < 0) j0 += a.nf;
   if (j0 >= a.nf) j0 -= a.nf;
-  int k;
-  if (q0 < 0 || q0 + a.w > a.nrow || !(a.theta[j] == a.theta[j]) || !(a.phi[j] == a.phi[j])) {
-    atomicAdd(bad, 1);
-    k = 0;
-  } else {
-    k = (q0 / a.tr) * a.ntc + (j0 / a.tc);
-  }
-  a.key[j] = k;
-  atomicAdd(&a.cnt[k], 1);
+  bad = q0 < 0 || q0 + a.w > a.nrow || !(theta == theta) || !(phi == phi);
+  return bad ? 0 : (q0 / a.tr) * a.ntc + (j0 / a.tc);
 }
 
-__global__ void k_bin_scatter(long npts, const double* __restrict__ theta, const double* __restrict__ phi,
-                              const int* __restrict__ key, const int* __restrict__ start, int* cursor,
-                              double2* __restrict__ pts, int* __restrict__ idx) {
+__global__ __launch_bounds__(256) void k_bin_count(BinArgs a, int* bad_count) {
   const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (j >= npts) return;
-  const int k = key[j];
-  const int pos = start[k] + atomicAdd(&cursor[k], 1);
+  const bool valid = j < a.npts;
+  bool bad = false;
+  int k = 0;
+  if (valid) {
+    k = bin_key(a, a.theta[j], a.phi[j], bad);
+    a.key[j] = k;
+  }
+  const unsigned long long bm = __ballot(valid && bad);
+  if ((threadIdx.x & 63) == 0 && bm) atomicAdd(bad_count, __popcll(bm));
+  const RunInfo r = wave_runs(k, valid);
+  if (valid && r.leader == (int)(threadIdx.x & 63)) atomicAdd(&a.cnt[k], r.len);
+}
+
+__global__ __launch_bounds__(256) void k_bin_scatter(long npts, const double* __restrict__ theta,
+                                                     const double* __restrict__ phi, const int* __restrict__ key,
+                                                     const int* __restrict__ start, int* cursor,
+                                                     double2* __restrict__ pts, int* __restrict__ idx) {
+  const long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const bool valid = j < npts;
+  const int k = valid ? key[j] : 0;
+  const RunInfo r = wave_runs(k, valid);
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  if (valid && r.leader == lane) base = start[k] + atomicAdd(&cursor[k], r.len);
+  base = __shfl(base, r.leader, 64);
+  if (!valid) return;
+  const int pos = base + (lane - r.leader);
   pts[pos] = double2{theta[j], phi[j]};
   idx[pos] = (int)j;
 }
@@ -620,10 +678,11 @@ void NufftPlan::type2_impl(const double2* G, long sm, const double* gamma, doubl
     NUSHT_LAUNCH_SIZED(k_phi2, T, p.nf, p.nrow, sizeof(T2) * (size_t)p.nf, pa, st);
   }
   if (npts_ == 0) return;
-  InterpArgs<T> ia{p.nf, p.w, p.row_lo, p.nrow, p.spin, (T)p.beta, npts_, pts_.as<double2>(), idx_.as<int>(),
-                   grid_.as<T2>(), gamma, ore, oim};
+  InterpArgs<T> ia{p.nf, p.w, p.row_lo, p.nrow, p.spin, tile_r_, tile_c_, ntile_c_, (T)p.beta, npts_,
+                   pts_.as<double2>(), idx_.as<int>(), tile_start_.as<int>(), grid_.as<T2>(), gamma, ore, oim};
   StageScope sc("nufft_interp", st);
-  hipLaunchKernelGGL(k_interp<T>, dim3(ceil_div(npts_, 256)), dim3(256), 0, st, ia);
+  const size_t tlds = sizeof(T2) * (size_t)(tile_r_ + p.w) * (tile_c_ + p.w);
+  hipLaunchKernelGGL(k_interp<T>, dim3(ntile_r_ * ntile_c_), dim3(256), tlds, st, ia);
   NUSHT_LAUNCH_CHECK();
 }
 

@@ -48,9 +48,10 @@ class RingPlan {
   DevBuf theta_, phi0_, ofs_;
   DevBuf chirp_, kern_, perm_in_, perm_out_;  // Bluestein / Rader tables
   DevBuf sqll_;                               // sqrt(l(l+1))
-  DevBuf G_;                                  // [nring][nrows] complex
+  DevBuf G_;                                  // [nrows][nring] complex (m-major)
   DevBuf coef_;
   int nrows_ = 0;
+  int gstride_ = 0;
 };
 
 }  // namespace nusht

@@ -19,7 +19,7 @@ constexpr int kRingNT = 640;
 constexpr int kRingNMAX = 8320;
 
 struct RingArgs {
-  int lmax, spin, nphi, nrows, method, mlen, mode;
+  int lmax, spin, nphi, nrows, gstride, method, mlen, mode;
   const double* theta;
   const double* phi0;
   const long* ofs;
@@ -102,19 +102,21 @@ template <int METHOD>
 __global__ __launch_bounds__(kRingNT) void k_ring(RingArgs a) {
   extern __shared__ unsigned char smem_raw[];
   double2* buf = reinterpret_cast<double2*>(smem_raw);
-  const int ring = blockIdx.x;
+  // neighbouring rings run on the same XCD at the same time, so the m-major G columns they read
+  // share that XCD's L2 lines
+  const int ring = xcd_remap(blockIdx.x, gridDim.x);
   const int n = a.nphi;
   const int L = a.lmax;
   const double ph0 = a.phi0[ring];
   const long ofs = a.ofs[ring];
-  double2* g = a.G + (long)ring * a.nrows;
+  double2* g = a.G + ring;  // G[row][ring], row stride gstride
   if (a.mode < 3) {
     for (int k = threadIdx.x; k < n; k += kRingNT) buf[k] = double2{0.0, 0.0};
     __syncthreads();
     // conjugated spectrum for the inverse DFT: X[m mod n] = conj(G_m e^{i m phi0})
     const int mlo = a.spin ? -L : 0;
     for (int m = mlo + (int)threadIdx.x; m <= L; m += kRingNT) {
-      double2 v = g[a.spin ? m + L : m];
+      double2 v = g[(long)(a.spin ? m + L : m) * a.gstride];
       if (ph0 != 0.0) {
         double s, c;
         sincos(m * ph0, &s, &c);
@@ -155,7 +157,7 @@ __global__ __launch_bounds__(kRingNT) void k_ring(RingArgs a) {
         sincos(-m * ph0, &s, &c);
         v = cmul(v, double2{c, s});
       }
-      g[a.spin ? m + L : m] = v;
+      g[(long)(a.spin ? m + L : m) * a.gstride] = v;
     }
   }
 }
@@ -289,12 +291,15 @@ RingPlan::RingPlan(int lmax, int spin, const std::vector<double>& theta, const s
   tab_.tag = "ring";
   tab_.build(lmax, spin, theta, st);
   nrows_ = spin ? 2 * lmax + 1 : lmax + 1;
-  G_.alloc(sizeof(double2) * (size_t)nring_ * nrows_);
+  // odd row stride: a power-of-two stride (e.g. 4096 rings) measurably slows the synthesis writes
+  gstride_ = nring_ | 1;
+  if (gstride_ == nring_) gstride_ += 2;
+  G_.alloc(sizeof(double2) * (size_t)gstride_ * nrows_);
   coef_.alloc(legendre_coef_bytes(tab_, 1));
   sqll_.alloc(sizeof(double) * (lmax + 1));
   hipLaunchKernelGGL(k_sqrt_ll, dim3(ceil_div(lmax + 1, 256)), dim3(256), 0, st, lmax, sqll_.as<double>());
   NUSHT_LAUNCH_CHECK();
-  legendre_autotune(tab_, spin ? 2 : 1, coef_.as<double2>(), G_.as<double2>(), 1, nrows_, st);
+  legendre_autotune(tab_, spin ? 2 : 1, coef_.as<double2>(), G_.as<double2>(), gstride_, 1, st);
 }
 
 size_t RingPlan::bytes() const {
@@ -304,7 +309,8 @@ size_t RingPlan::bytes() const {
 
 void RingPlan::ring_pass(int mode, double* o0, double* o1, double* o2, hipStream_t st) {
   RingArgs a;
-  a.lmax = lmax_; a.spin = spin_; a.nphi = nphi_; a.nrows = nrows_; a.method = (int)method_; a.mlen = mlen_;
+  a.lmax = lmax_; a.spin = spin_; a.nphi = nphi_; a.nrows = nrows_; a.gstride = gstride_; a.method = (int)method_;
+  a.mlen = mlen_;
   a.mode = mode;
   a.theta = theta_.as<double>(); a.phi0 = phi0_.as<double>(); a.ofs = ofs_.as<long>();
   a.G = G_.as<double2>();
@@ -325,18 +331,18 @@ void RingPlan::ring_pass(int mode, double* o0, double* o1, double* o2, hipStream
 }
 
 void RingPlan::synthesis(const double2* alm, int ncomp, double* map0, double* map1, hipStream_t st) {
-  legendre_synth(tab_, 1, ncomp, alm, 0, nullptr, coef_.as<double2>(), G_.as<double2>(), 1, nrows_, 0, st);
+  legendre_synth(tab_, 1, ncomp, alm, 0, nullptr, coef_.as<double2>(), G_.as<double2>(), gstride_, 1, 0, st);
   ring_pass(spin_ ? 1 : 0, map0, map1, nullptr, st);
 }
 
 void RingPlan::adjoint(const double* map0, const double* map1, double2* alm, int ncomp, hipStream_t st) {
   ring_pass(3, const_cast<double*>(map0), const_cast<double*>(map1), nullptr, st);
-  legendre_adjoint(tab_, ncomp, G_.as<double2>(), 1, nrows_, alm, st);
+  legendre_adjoint(tab_, ncomp, G_.as<double2>(), gstride_, 1, alm, st);
 }
 
 void RingPlan::dlm2pointing(const double2* dlm_scaled, double* ptheta, double* pphi, double* gamma, hipStream_t st) {
   NUSHT_REQUIRE(spin_ == 1, kErrState, "dlm2pointing needs a spin-1 ring plan");
-  legendre_synth(tab_, 1, 1, dlm_scaled, 0, sqll_.as<double>(), coef_.as<double2>(), G_.as<double2>(), 1, nrows_, 0,
+  legendre_synth(tab_, 1, 1, dlm_scaled, 0, sqll_.as<double>(), coef_.as<double2>(), G_.as<double2>(), gstride_, 1, 0,
                  st);
   ring_pass(2, ptheta, pphi, gamma, st);
 }
