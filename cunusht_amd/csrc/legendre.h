@@ -23,7 +23,8 @@ struct LegendreTables {
   int lmax = 0;
   int spin = 0;
   int nslot = 0;          // ring pairs + unpaired rings
-  int sstride = 0;        // row stride of the seed table (odd, >= nslot)
+  int sstride = 0;        // row stride of the state tables (odd, >= nslot)
+  int ngroups = 0;        // groups of 64 consecutive slots
   int nring = 0;
   DevBuf slot_theta;      // double[nslot]   theta of the slot's northern ring
   DevBuf slot_north;      // int[nslot]      ring index
@@ -31,7 +32,12 @@ struct LegendreTables {
   DevBuf mofs;            // int64[lmax+2]   offset of m's l-range in the (l, m) tables
   DevBuf rec;             // double2[nlm]    (A_l, B_l)
   DevBuf cnorm;           // double[nlm]     c_l
-  DevBuf seeds;           // double4[(lmax+1) * nslot]  (mu_u, mu_v, k_u, k_v) at l0 = max(m, s)
+  // Recursion state per (m, slot) at the first l its 64-slot group needs: every l below it has
+  // all values of the group under 2^-300 (they would add only zeros).  start - l0 is a multiple of 8.
+  DevBuf st_mu;           // double2[(lmax+1) * sstride]  (mu_u, mu_v) at l = start
+  DevBuf st_p;            // double2[(lmax+1) * sstride]  (mu_u, mu_v) at l = start - 1 (0 at l0)
+  DevBuf st_k;            // double2[(lmax+1) * sstride]  scale exponents (k_u, k_v) at l = start
+  DevBuf start;           // int[(lmax+1) * ngroups]
   long nlm = 0;           // sum over m of (lmax - l0(m) + 1)
   std::string tag = "rings";  // stage-timer label ("cc" for the nuSHT grid, "ring" for the pointing grid)
   std::string n_prep, n_synth, n_adj;
@@ -42,7 +48,7 @@ struct LegendreTables {
   void build(int lmax, int spin, const std::vector<double>& ring_theta, hipStream_t st);
   size_t bytes() const {
     return slot_theta.bytes() + slot_north.bytes() + slot_south.bytes() + mofs.bytes() + rec.bytes() +
-           cnorm.bytes() + seeds.bytes();
+           cnorm.bytes() + st_mu.bytes() + st_p.bytes() + st_k.bytes() + start.bytes();
   }
 };
 

@@ -64,7 +64,8 @@ __device__ inline void renorm(double& v, double& k) {
 
 // Starting values at l0 = max(m, s): u = N_l0 d^l0_{m,-s}, v = N_l0 d^l0_{m,+s}, with scale exponents.
 __global__ __launch_bounds__(64) void k_seeds(int lmax, int s, int nslot, int sstride,
-                                              const double* __restrict__ slot_theta, double4* __restrict__ seeds) {
+                                              const double* __restrict__ slot_theta, double2* __restrict__ st_mu,
+                                              double2* __restrict__ st_p, double2* __restrict__ st_k) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nslot) return;
   const double th = slot_theta[i];
@@ -78,7 +79,10 @@ __global__ __launch_bounds__(64) void k_seeds(int lmax, int s, int nslot, int ss
     double uu = ns * bin * pow(ch, (double)(s - m)) * pow(sh, (double)(s + m));
     if ((m + s) & 1) uu = -uu;
     double vv = ns * bin * pow(ch, (double)(s + m)) * pow(sh, (double)(s - m));
-    seeds[(long)m * sstride + i] = double4{uu, vv, 0.0, 0.0};
+    const long q = (long)m * sstride + i;
+    st_mu[q] = double2{uu, vv};
+    st_p[q] = double2{0.0, 0.0};
+    st_k[q] = double2{0.0, 0.0};
   }
   if (s > lmax) return;
   // m = s: u = N_s sin^{2s}(th/2), v = N_s cos^{2s}(th/2); then the product recursion in m.
@@ -86,15 +90,79 @@ __global__ __launch_bounds__(64) void k_seeds(int lmax, int s, int nslot, int ss
   for (int j = 0; j < 2 * s; ++j) { u *= sh; v *= ch; }
   renorm(u, ku);
   renorm(v, kv);
-  seeds[(long)s * sstride + i] = double4{u, v, ku, kv};
+  {
+    const long q = (long)s * sstride + i;
+    st_mu[q] = double2{u, v};
+    st_p[q] = double2{0.0, 0.0};
+    st_k[q] = double2{ku, kv};
+  }
   for (int m = s + 1; m <= lmax; ++m) {
     double f = -sqrt(2.0 * m * (2.0 * m + 1) / ((double)m * m - (double)s * s)) * sh * ch;
     u *= f;
     v *= f;
     renorm(u, ku);
     renorm(v, kv);
-    seeds[(long)m * sstride + i] = double4{u, v, ku, kv};
+    const long q = (long)m * sstride + i;
+    st_mu[q] = double2{u, v};
+    st_p[q] = double2{0.0, 0.0};
+    st_k[q] = double2{ku, kv};
   }
+}
+
+// Advance each (m, 64-slot group) through the l-range in which every value of the group is
+// below 2^-300, in steps of kStartAlign, and store the state at the first l that may contribute.
+// Scale exponents never decrease, so a group that is still all-under after a step of kStartAlign
+// was all-under at every l inside it; otherwise the step is undone.
+constexpr int kStartAlign = 8;
+
+template <bool SP>
+__global__ __launch_bounds__(64) void k_advance(int lmax, int s, int nslot, int sstride, int ngroups,
+                                                const double* __restrict__ slot_theta, const long* __restrict__ mofs,
+                                                const double2* __restrict__ rec_all, double2* __restrict__ st_mu,
+                                                double2* __restrict__ st_p, double2* __restrict__ st_k,
+                                                int* __restrict__ start) {
+  const int m = blockIdx.x / ngroups;
+  const int g = blockIdx.x - m * ngroups;
+  const int l0 = l0_of(m, s);
+  if (l0 > lmax) return;
+  const int slot = g * 64 + (int)threadIdx.x;
+  const bool valid = slot < nslot;
+  const long q = (long)m * sstride + slot;
+  double x = 0, mu = 0, mv = 0, pu = 0, pv = 0, ku = 0, kv = 0;
+  if (valid) {
+    x = cos(slot_theta[slot]);
+    const double2 a = st_mu[q], k = st_k[q];
+    mu = a.x; ku = k.x;
+    if (SP) { mv = a.y; kv = k.y; }
+  }
+  auto dead = [&]() { return !__any(valid && (ku == 0.0 || (SP && kv == 0.0))); };
+  const double2* __restrict__ rec = rec_all + mofs[m];
+  int lc = l0;
+  while (lc + kStartAlign - 1 <= lmax && dead()) {
+    const double smu = mu, smv = mv, spu = pu, spv = pv, sku = ku, skv = kv;
+#pragma unroll
+    for (int i = 0; i < kStartAlign; ++i) {
+      const double2 rc = rec[lc - l0 + i];
+      const double t = rc.x * x;
+      const double nu = fma(t + rc.y, mu, -pu);
+      pu = mu; mu = nu;
+      if (SP) {
+        const double nv = fma(t - rc.y, mv, -pv);
+        pv = mv; mv = nv;
+      }
+      if (ku < 0.0 && fabs(mu) > kThresh) { mu *= kSmall; pu *= kSmall; ku += 1.0; }
+      if (SP && kv < 0.0 && fabs(mv) > kThresh) { mv *= kSmall; pv *= kSmall; kv += 1.0; }
+    }
+    if (dead()) { lc += kStartAlign; continue; }
+    mu = smu; mv = smv; pu = spu; pv = spv; ku = sku; kv = skv;
+    break;
+  }
+  if (valid && lc > l0) {
+    st_mu[q] = double2{mu, mv};
+    st_p[q] = double2{pu, pv};
+    st_k[q] = double2{ku, kv};
+  }
+  if (threadIdx.x == 0) start[(long)m * ngroups + g] = lc;
 }
 
 // ------------------------------------------------------------------ synthesis prep
@@ -152,40 +220,47 @@ struct SynthArgs {
   const int* slot_south;
   const long* mofs;
   const double2* rec;
-  const double4* seeds;
+  const double2* st_mu;
+  const double2* st_p;
+  const double2* st_k;
+  const int* start;
   const double2* coef;
   double2* G;
   long sm, sr, smap;
 };
 
-// R ring pairs per lane (independent recursion chains); the l-loop runs in chunks of CH so
-// that the wave-uniform coefficients of a chunk are fetched together (one scalar-cache round
-// trip per chunk instead of per l).  Tables are padded by CH entries so a chunk never reads
-// past the end.
+// One wave per (m, group of 64 ring pairs), starting at the group's first contributing l.
+// The l-loop runs in chunks of CH so that the wave-uniform coefficients of a chunk are fetched
+// together (one scalar-cache round trip per chunk instead of per l).  Tables are padded by CH
+// entries so a chunk never reads past the end.
 constexpr int kSynCH = 8;  // largest chunk of any variant (table padding)
 
-template <bool SP, int NB, int R, int CH, int WPE>
+template <bool SP, int NB, int CH, int WPE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void k_synth(SynthArgs a) {
+  constexpr int R = 1;
   constexpr int K = SP ? 2 * NB : NB;
   const int blk = blockIdx.x;
   const int m = blk / a.nsb;
   const int sb = blk - m * a.nsb;
   const int L = a.lmax;
   const int l0 = l0_of(m, a.s);
-  if (l0 > L) return;
+  // the launch may carry one empty slot block per m (odd block count per m, see launch_synth)
+  if (l0 > L || sb * 64 >= a.nslot) return;
   const int lane = threadIdx.x;
+  const int lstart = a.start[(long)m * ((a.nslot + 63) >> 6) + sb];
 
   double x[R], mu_u[R], mu_v[R], pu[R], pv[R], ku[R], kv[R];
   int slot[R];
 #pragma unroll
   for (int r = 0; r < R; ++r) {
-    slot[r] = sb * 64 * R + r * 64 + lane;
+    slot[r] = sb * 64 + lane;
     x[r] = mu_u[r] = mu_v[r] = pu[r] = pv[r] = ku[r] = kv[r] = 0.0;
     if (slot[r] < a.nslot) {
+      const long q = (long)m * a.sstride + slot[r];
       x[r] = cos(a.slot_theta[slot[r]]);
-      double4 sd = a.seeds[(long)m * a.sstride + slot[r]];
-      mu_u[r] = sd.x; ku[r] = sd.z;
-      if (SP) { mu_v[r] = sd.y; kv[r] = sd.w; }
+      const double2 sm = a.st_mu[q], sp = a.st_p[q], sk = a.st_k[q];
+      mu_u[r] = sm.x; pu[r] = sp.x; ku[r] = sk.x;
+      if (SP) { mu_v[r] = sm.y; pv[r] = sp.y; kv[r] = sk.y; }
     }
   }
 
@@ -256,7 +331,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
   };
   using T_ = std::true_type;
   using F_ = std::false_type;
-  int lc = l0;
+  int lc = lstart;
   // slow phase: while some lane of the wave is still below 2^-300
   for (; lc <= L; lc += CH) {
     bool under = false;
@@ -356,14 +431,17 @@ __device__ __forceinline__ int reduce_index() {
 }
 
 struct AdjArgs {
-  int lmax, s, nslot, sstride, ncomp;
+  int lmax, s, nslot, sstride, ngroups, ncomp;
   const double* slot_theta;
   const int* slot_north;
   const int* slot_south;
   const long* mofs;
   const double2* rec;
   const double* cnorm;
-  const double4* seeds;
+  const double2* st_mu;
+  const double2* st_p;
+  const double2* st_k;
+  const int* start;
   const double2* G;
   long sm, sr;
   double2* alm;
@@ -397,22 +475,31 @@ __global__ __launch_bounds__(AdjCfg<SP>::MaxThreads) void k_adjoint(AdjArgs a) {
   const bool writer = ((threadIdx.x & 63) & (64 / K - 1)) == 0;
   const int nthr = blockDim.x;
   const int cap = nthr * kAdjR;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int lane = threadIdx.x & 63;
 
   for (int pass0 = 0; pass0 < a.nslot; pass0 += cap) {
-  // per-lane ring pairs
+  // per-lane ring pairs: the wave takes kAdjR adjacent groups of 64 ring pairs; group r joins
+  // the l-loop at its first contributing l, lst[r]
   double x[kAdjR], mu[kAdjR], pm[kAdjR], ku[kAdjR];
   double nv_[kAdjR], pv[kAdjR], kv[kAdjR];
   double2 P0[kAdjR], P1[kAdjR], Q0[kAdjR], Q1[kAdjR];  // spin 0: E/O in P0/P1; spin s: PN, PS, QN, QS
+  int lst[kAdjR];
+  int lmin = L + 1;
 #pragma unroll
   for (int r = 0; r < kAdjR; ++r) {
-    const int slot = pass0 + r * nthr + threadIdx.x;
+    const int g = (pass0 >> 6) + wave * kAdjR + r;
+    const int slot = g * 64 + lane;
+    lst[r] = g < a.ngroups ? a.start[(long)m * a.ngroups + g] : L + 1;
+    lmin = min(lmin, lst[r]);
     x[r] = mu[r] = pm[r] = ku[r] = nv_[r] = pv[r] = kv[r] = 0.0;
     P0[r] = P1[r] = Q0[r] = Q1[r] = double2{0.0, 0.0};
     if (slot < a.nslot) {
+      const long q = (long)m * a.sstride + slot;
       x[r] = cos(a.slot_theta[slot]);
-      double4 sd = a.seeds[(long)m * a.sstride + slot];
-      mu[r] = sd.x; ku[r] = sd.z;
-      if (SP) { nv_[r] = sd.y; kv[r] = sd.w; }
+      const double2 sm = a.st_mu[q], sp = a.st_p[q], sk = a.st_k[q];
+      mu[r] = sm.x; pm[r] = sp.x; ku[r] = sk.x;
+      if (SP) { nv_[r] = sm.y; pv[r] = sp.y; kv[r] = sk.y; }
       const int rn = a.slot_north[slot];
       int rs = a.slot_south[slot];
       if (rs == rn) rs = -1;
@@ -430,15 +517,20 @@ __global__ __launch_bounds__(AdjCfg<SP>::MaxThreads) void k_adjoint(AdjArgs a) {
       }
     }
   }
-  for (int lc = l0; lc <= L; lc += K) {
+  // group starts are l0 plus multiples of 8, so they fall on chunk boundaries
+  for (int lc = lmin; lc <= L; lc += K) {
     double2 X[K], Y[K];
 #pragma unroll
     for (int i = 0; i < K; ++i) X[i] = double2{0.0, 0.0};
 #pragma unroll
     for (int i = 0; i < K; ++i) Y[i] = double2{0.0, 0.0};
     bool slow = false;
+    bool act[kAdjR];
 #pragma unroll
-    for (int r = 0; r < kAdjR; ++r) slow |= (ku[r] < 0.0) || (kv[r] < 0.0);
+    for (int r = 0; r < kAdjR; ++r) {
+      act[r] = lc >= lst[r];
+      slow |= act[r] && ((ku[r] < 0.0) || (kv[r] < 0.0));
+    }
     slow = __any(slow);
 #pragma unroll
     for (int i = 0; i < K; ++i) {
@@ -448,6 +540,7 @@ __global__ __launch_bounds__(AdjCfg<SP>::MaxThreads) void k_adjoint(AdjArgs a) {
         const double2 rc = (l < L) ? rec[l - l0] : double2{0.0, 0.0};
 #pragma unroll
         for (int r = 0; r < kAdjR; ++r) {
+          if (!act[r]) continue;
           double wu = mu[r], wv = nv_[r];
           if (slow) { wu = ku[r] == 0.0 ? wu : 0.0; wv = kv[r] == 0.0 ? wv : 0.0; }
           if (!SP) {
@@ -570,16 +663,28 @@ void LegendreTables::build(int L, int s, const std::vector<double>& theta, hipSt
   // padded so that a chunked l-loop may read up to kSynCH entries past the last one
   rec.alloc(sizeof(double2) * (size_t)(std::max<long>(nlm, 1) + kSynCH));
   cnorm.alloc(sizeof(double) * (size_t)(std::max<long>(nlm, 1) + kSynCH));
-  // seed rows are padded to an odd number of entries: a power-of-two row stride (2048 ring pairs
-  // = 64 KiB) measurably slows the synthesis
-  sstride = nslot | 1;
-  if (sstride == nslot) sstride += 2;
-  seeds.alloc(sizeof(double4) * (size_t)(L + 1) * sstride);
+  sstride = nslot;
+  ngroups = ceil_div(nslot, 64);
+  const size_t nst = (size_t)(L + 1) * sstride;
+  st_mu.alloc(sizeof(double2) * nst);
+  st_p.alloc(sizeof(double2) * nst);
+  st_k.alloc(sizeof(double2) * nst);
+  start.alloc(sizeof(int) * (size_t)(L + 1) * ngroups);
   hipLaunchKernelGGL(k_rec_tables, dim3(ceil_div(L + 1, 128)), dim3(128), 0, st, L, s, mofs.as<long>(),
                      rec.as<double2>(), cnorm.as<double>());
   NUSHT_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_seeds, dim3(ceil_div(nslot, 64)), dim3(64), 0, st, L, s, nslot, sstride,
-                     slot_theta.as<double>(), seeds.as<double4>());
+                     slot_theta.as<double>(), st_mu.as<double2>(), st_p.as<double2>(), st_k.as<double2>());
+  NUSHT_LAUNCH_CHECK();
+  const dim3 grid((unsigned)((long)ngroups * (L + 1)));
+  if (s > 0)
+    hipLaunchKernelGGL(k_advance<true>, grid, dim3(64), 0, st, L, s, nslot, sstride, ngroups,
+                       slot_theta.as<double>(), mofs.as<long>(), rec.as<double2>(), st_mu.as<double2>(),
+                       st_p.as<double2>(), st_k.as<double2>(), start.as<int>());
+  else
+    hipLaunchKernelGGL(k_advance<false>, grid, dim3(64), 0, st, L, s, nslot, sstride, ngroups,
+                       slot_theta.as<double>(), mofs.as<long>(), rec.as<double2>(), st_mu.as<double2>(),
+                       st_p.as<double2>(), st_k.as<double2>(), start.as<int>());
   NUSHT_LAUNCH_CHECK();
 }
 
@@ -590,26 +695,26 @@ size_t legendre_coef_bytes(const LegendreTables& t, int nmaps) {
   return sizeof(double2) * (size_t)(std::max<long>(t.nlm, 1) + kSynCH) * padded_maps(nmaps) * (t.spin ? 2 : 1);
 }
 
-// Synthesis variants (ring pairs per lane R, l-chunk CH, minimum waves per SIMD); the
-// autotuner picks one per table.
+// Synthesis variants (l-chunk CH, minimum waves per SIMD); the autotuner picks one per table.
 struct SynVariant {
-  int R, CH, WPE;
+  int CH, WPE;
 };
-static const SynVariant kSynVariants[] = {{1, 4, 1}, {1, 8, 1}, {2, 2, 1}, {2, 4, 1}, {1, 2, 1}};
-constexpr int kNumSynVariants = 5;
+static const SynVariant kSynVariants[] = {{2, 1}, {4, 1}, {8, 1}};
+constexpr int kNumSynVariants = 3;
 
 template <bool SP, int NB>
 static void launch_synth(const SynthArgs& a, int variant, hipStream_t st) {
-  const SynVariant v = kSynVariants[variant];
-  const int nblocks = ceil_div(a.nslot, 64 * v.R) * (a.lmax + 1);
+  // Odd number of slot blocks per m: blocks are handed to XCDs and shader engines round-robin, and
+  // with a power-of-two count every engine would see the same slot blocks for every m, so the
+  // polar blocks (long underflow phases) would pile up on a few engines.
+  const int nsb = ceil_div(a.nslot, 64) | 1;
+  const int nblocks = nsb * (a.lmax + 1);
   SynthArgs b = a;
-  b.nsb = ceil_div(a.nslot, 64 * v.R);
+  b.nsb = nsb;
   switch (variant) {
-    case 0: hipLaunchKernelGGL((k_synth<SP, NB, 1, 4, 1>), dim3(nblocks), dim3(64), 0, st, b); break;
-    case 1: hipLaunchKernelGGL((k_synth<SP, NB, 1, 8, 1>), dim3(nblocks), dim3(64), 0, st, b); break;
-    case 2: hipLaunchKernelGGL((k_synth<SP, NB, 2, 2, 1>), dim3(nblocks), dim3(64), 0, st, b); break;
-    case 3: hipLaunchKernelGGL((k_synth<SP, NB, 2, 4, 1>), dim3(nblocks), dim3(64), 0, st, b); break;
-    default: hipLaunchKernelGGL((k_synth<SP, NB, 1, 2, 1>), dim3(nblocks), dim3(64), 0, st, b); break;
+    case 0: hipLaunchKernelGGL((k_synth<SP, NB, 2, 1>), dim3(nblocks), dim3(64), 0, st, b); break;
+    case 1: hipLaunchKernelGGL((k_synth<SP, NB, 4, 1>), dim3(nblocks), dim3(64), 0, st, b); break;
+    default: hipLaunchKernelGGL((k_synth<SP, NB, 8, 1>), dim3(nblocks), dim3(64), 0, st, b); break;
   }
   NUSHT_LAUNCH_CHECK();
 }
@@ -648,11 +753,14 @@ static void legendre_synth_chunk(const LegendreTables& t, int nmaps, int ncomp, 
   a.slot_south = t.slot_south.as<int>();
   a.mofs = t.mofs.as<long>();
   a.rec = t.rec.as<double2>();
-  a.seeds = t.seeds.as<double4>();
+  a.st_mu = t.st_mu.as<double2>();
+  a.st_p = t.st_p.as<double2>();
+  a.st_k = t.st_k.as<double2>();
+  a.start = t.start.as<int>();
   a.coef = coef_ws;
   a.G = G; a.sm = sm; a.sr = sr; a.smap = smap;
   const bool sp = t.spin > 0;
-  const int var = (t.variant >= 0 && t.variant < kNumSynVariants) ? t.variant : 3;
+  const int var = (t.variant >= 0 && t.variant < kNumSynVariants) ? t.variant : 0;
   StageScope sc(t.n_synth.c_str(), st);
   switch (nbpad) {  // the kernel's NB must equal the coefficient stride written by k_prep
     case 1: sp ? launch_synth<true, 1>(a, var, st) : launch_synth<false, 1>(a, var, st); break;
@@ -678,7 +786,9 @@ void legendre_autotune(LegendreTables& t, int ncomp, double2* coef_ws, double2* 
   NUSHT_HIP(hipEventCreate(&e0));
   NUSHT_HIP(hipEventCreate(&e1));
   float best = 1e30f;
-  int bestv = 3;
+  int bestv = 0;
+  const char* env = getenv("NUSHT_VERBOSE");
+  const bool verbose = env && env[0] && env[0] != '0';
   for (int v = 0; v < kNumSynVariants; ++v) {
     t.variant = v;
     legendre_synth(t, 1, ncomp, alm, 0, nullptr, coef_ws, G, sm, sr, 0, st);  // warm-up
@@ -688,6 +798,8 @@ void legendre_autotune(LegendreTables& t, int ncomp, double2* coef_ws, double2* 
     NUSHT_HIP(hipEventSynchronize(e1));
     float ms = 0;
     NUSHT_HIP(hipEventElapsedTime(&ms, e0, e1));
+    if (verbose) fprintf(stderr, "[nusht] legendre autotune %s lmax %d spin %d pairs %d: variant %d (CH %d) %.3f ms\n",
+                         t.tag.c_str(), t.lmax, t.spin, t.nslot, v, kSynVariants[v].CH, ms / 2);
     if (ms < best) { best = ms; bestv = v; }
   }
   t.variant = bestv;
@@ -702,14 +814,17 @@ void legendre_adjoint(const LegendreTables& t, int ncomp, const double2* G, long
                       hipStream_t st) {
   const int L = t.lmax;
   AdjArgs a;
-  a.lmax = L; a.s = t.spin; a.nslot = t.nslot; a.sstride = t.sstride; a.ncomp = ncomp;
+  a.lmax = L; a.s = t.spin; a.nslot = t.nslot; a.sstride = t.sstride; a.ngroups = t.ngroups; a.ncomp = ncomp;
   a.slot_theta = t.slot_theta.as<double>();
   a.slot_north = t.slot_north.as<int>();
   a.slot_south = t.slot_south.as<int>();
   a.mofs = t.mofs.as<long>();
   a.rec = t.rec.as<double2>();
   a.cnorm = t.cnorm.as<double>();
-  a.seeds = t.seeds.as<double4>();
+  a.st_mu = t.st_mu.as<double2>();
+  a.st_p = t.st_p.as<double2>();
+  a.st_k = t.st_k.as<double2>();
+  a.start = t.start.as<int>();
   a.G = G; a.sm = sm; a.sr = sr; a.alm = alm;
   const int R = t.spin ? AdjCfg<true>::R : AdjCfg<false>::R;
   const int maxw = (t.spin ? AdjCfg<true>::MaxThreads : AdjCfg<false>::MaxThreads) / 64;

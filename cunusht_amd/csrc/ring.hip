@@ -291,9 +291,7 @@ RingPlan::RingPlan(int lmax, int spin, const std::vector<double>& theta, const s
   tab_.tag = "ring";
   tab_.build(lmax, spin, theta, st);
   nrows_ = spin ? 2 * lmax + 1 : lmax + 1;
-  // odd row stride: a power-of-two stride (e.g. 4096 rings) measurably slows the synthesis writes
-  gstride_ = nring_ | 1;
-  if (gstride_ == nring_) gstride_ += 2;
+  gstride_ = nring_;
   G_.alloc(sizeof(double2) * (size_t)gstride_ * nrows_);
   coef_.alloc(legendre_coef_bytes(tab_, 1));
   sqll_.alloc(sizeof(double) * (lmax + 1));

@@ -178,6 +178,70 @@ def test_dlm2pointing_vs_oracle(L, nphi):
     assert np.max(np.abs(got[2] - g)) < 1e-10
 
 
+@pytest.mark.parametrize("spin", [0, 1, 2])
+@pytest.mark.parametrize("mcut", [0, 40, 120])
+def test_ring_sht_polar_underflow(spin, mcut):
+    """Rings 1e-4 .. 0.05 rad from both poles at lmax 320, alm zero for m < mcut: at such rings the
+    Wigner-d values of large m start far below 2^-300 (scaled recursion, l-ranges that are skipped
+    because a whole group of rings is still under range, partly masked groups).  Each ring's
+    values and each m's alm are compared with the oracle relative to their own maximum; values
+    below 1e-85 of the global maximum are exempt (the library drops terms below 2^-300 = 4.9e-91
+    of the O(1) recursion values).  A southern ring theta = fl(pi - eps) fixes eps only to ulp(pi),
+    and values ~ eps^m change by m ulp(theta) / sin(theta) relative to it; the library uses the
+    exact mirror of the northern ring, so each ring's tolerance adds L ulp(theta) / sin(theta)."""
+    from cunusht_amd.geometry import Geom
+    from cunusht_amd.transformer import RingSHT
+    L, nphi = 320, 648
+    rng = np.random.default_rng(1000 + 10 * spin + mcut)
+    polar = np.geomspace(1e-4, 0.05, 150)
+    theta = np.concatenate([polar, np.pi - polar[::2], np.linspace(0.3, 2.8, 37)])
+    nring = theta.size
+    phi0 = rng.uniform(0, 2 * np.pi, nring)
+    geom = Geom(theta, phi0, np.full(nring, nphi), np.arange(nring) * nphi, np.ones(nring))
+    r = RingSHT(geom, L, spin)
+    nc = 1 if spin == 0 else 2
+    a = almutil.random_alm(L, spin, nc, rng)
+    mofa = np.concatenate([np.full(L + 1 - m, m) for m in range(L + 1)])  # m of each alm index
+    a[:, mofa < mcut] = 0
+    npix = nring * nphi
+    out = torch.zeros((2, npix), dtype=torch.float64, device="cuda")
+    lib = _lib()
+    ad = _t(a)
+    lib.call("nusht_ringplan_synthesis", r._h, _p(ad), nc, _p(out[0]), _p(out[1]) if spin else None, _stream())
+    torch.cuda.synchronize()
+    G = clib.sht_synth_rings(a, L, spin, theta)
+    ref = pipeline.phi_synth(G, L, spin, nphi, phi0)
+    got = out.cpu().numpy()
+    got = got[0].reshape(nring, nphi) if spin == 0 else (got[0] + 1j * got[1]).reshape(nring, nphi)
+    floor = 1e-85 * np.max(np.abs(ref))
+    rmax = np.max(np.abs(ref), axis=1)
+    err = np.max(np.abs(got - ref), axis=1)
+    cond = L * np.spacing(theta) / np.sin(theta)
+    bad = np.nonzero(err > (1e-12 + cond) * rmax + floor)[0]
+    assert bad.size == 0, [(int(i), theta[i], rmax[i], err[i]) for i in bad[:8]]
+    # adjoint with values on the polar rings only (southern ones with eps >= 1e-2, where the
+    # conditioning term stays below 1.5e-11)
+    x = np.zeros((2, nring, nphi))
+    sel = (np.arange(nring) < 225) & ((theta < 1.0) | (np.pi - theta >= 1e-2))
+    x[:, sel] = rng.standard_normal((2, int(sel.sum()), nphi))
+    if spin == 0:
+        x[1] = 0
+    xd = _t(x.reshape(2, -1))
+    res = torch.zeros((nc, (L + 1) * (L + 2) // 2), dtype=torch.complex128, device="cuda")
+    lib.call("nusht_ringplan_adjoint", r._h, _p(xd[0]), _p(xd[1]) if spin else None, _p(res), nc, _stream())
+    _sync()
+    xm = x[0] if spin == 0 else x[0] + 1j * x[1]
+    Gt = pipeline.phi_adjoint(xm, L, spin, phi0)
+    aref = clib.sht_adjoint_rings(Gt, L, spin, nc, theta)
+    agot = res.cpu().numpy()
+    floor = 1e-85 * np.max(np.abs(aref))
+    for m in range(L + 1):
+        sl = mofa == m
+        am = np.max(np.abs(aref[:, sl]))
+        e = np.max(np.abs(agot[:, sl] - aref[:, sl]))
+        assert e <= 2e-11 * am + floor, (m, am, e)
+
+
 # ------------------------------------------------------------------ nusht2d2 / nusht2d1
 @pytest.mark.parametrize("spin", [0, 2])
 @pytest.mark.parametrize("eps", [1e-12, 1e-7, 1e-5])
