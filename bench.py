@@ -51,19 +51,21 @@ def synth_inputs(lmax: int, rng: np.random.Generator):
     return np.stack([elm, blm]), dlm_scaled
 
 
-def stage_work(stage: str, info: dict, lmax: int, npix: int, maps_per_launch: int = 1):
-    """(flops or bytes, unit) of one launch of a stage -- the algorithmic work defined in DESIGN.md.
+def stage_work(stage: str, info: dict, lmax: int, npix: int, maps_per_launch: int = 1, rinfo: dict | None = None):
+    """(flops or bytes, unit) of one launch of a stage.
 
-    Spin-s Legendre synthesis: per (l, m, ring pair) 7 flops of recursion (shared by the maps of one
-    launch) plus 16 flops of contraction per map (4 complex-by-real FMAs: +m/-m, north/south)."""
-    nlm = (lmax + 1) * (lmax + 2) // 2
+    Spin-s Legendre synthesis: per (l, m, ring pair) step that the kernel runs, 7 flops of
+    recursion (shared by the maps of one launch) plus 16 flops of contraction per map (4
+    complex-by-real FMAs: +m/-m, north/south).  The step count is the plan's
+    legendre_pair_steps: l-ranges in which a whole group of 64 ring pairs is below 2^-300 are
+    skipped, so they are not counted."""
     nt = info["ntheta"]
     single = info["single_precision"]
     bc = 8 if single else 16
     if stage == "legendre_synth_cc_s2":
-        return nlm * math.ceil(nt / 2) * (7 + 16 * maps_per_launch), "flop"
-    if stage == "legendre_synth_ring_s1":
-        return nlm * math.ceil((lmax + 1) / 2) * (7 + 16), "flop"
+        return info["legendre_pair_steps"] * (7 + 16 * maps_per_launch), "flop"
+    if stage == "legendre_synth_ring_s1" and rinfo is not None:
+        return rinfo["legendre_pair_steps"] * (7 + 16), "flop"
     if stage == "nufft_interp":
         return npix * (16 + 16) + info["nrow_fine"] * info["nfine"] * bc, "byte"
     if stage == "theta_pass_t2":
@@ -126,10 +128,13 @@ def main():
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-lmax", type=int, default=1023)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after the timed region, copy a buffer of this many MiB once (a plain copy kernel "
+                         "of known traffic, used to calibrate FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: every box in this setup has a single MI355X; "
-              "requests shard across GPUs without communication (see DESIGN.md)", file=sys.stderr)
+        print("bench.py measures one GPU only (N=1). Independent lensing requests shard across GPUs "
+              "without communication, so no multi-GPU figure is produced here.", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -187,6 +192,8 @@ def main():
     torch.cuda.synchronize()
     print("[bench] warmup done", file=sys.stderr, flush=True)
     info = plan.info()
+    rinfo = ring.info()
+    nlm = (L + 1) * (L + 2) // 2
 
     # plain: per-call latency with events on the launch stream
     _lib.timing_reset()
@@ -223,7 +230,7 @@ def main():
     name, (tot_ms, calls) = dom
     launches_cc = stages_batch.get("legendre_synth_cc_s2", (0, 1))[1]
     mpl = B * args.steps / max(launches_cc, 1) if name == "legendre_synth_cc_s2" else 1
-    work, unit = stage_work(name, info, L, npix, mpl)
+    work, unit = stage_work(name, info, L, npix, mpl, rinfo)
     roof = None
     if work is not None:
         avg_s = tot_ms / calls / 1e3
@@ -243,6 +250,15 @@ def main():
         w_i, _ = stage_work("nufft_interp", info, L, npix)
         a = w_i / (interp[0] / interp[1] / 1e3) / 1e9
         nufft_roof = {"achieved_GBs": a, "frac": a / PEAK_HBM_GBS, "avg_ms": interp[0] / interp[1]}
+
+    if args.copy_probe_mib > 0:
+        n = args.copy_probe_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     cpu = None
     if not args.no_cpu:
@@ -267,7 +283,11 @@ def main():
                    "lmax": L, "epsilon": eps, "batch": B, "npix": npix,
                    "nufft": {"nfine": info["nfine"], "nrow": info["nrow_fine"], "w": info["kernel_width"],
                              "sigma": round(info["sigma"], 4)},
-                   "legendre_variant": info["legendre_variant"]},
+                   "legendre_variant": info["legendre_variant"],
+                   "legendre_pair_steps": {"nusht_cc": info["legendre_pair_steps"],
+                                           "full_cc": nlm * math.ceil(info["ntheta"] / 2),
+                                           "pointing_gl": rinfo["legendre_pair_steps"],
+                                           "full_gl": nlm * math.ceil((L + 1) / 2)}},
         "plain": {"value": rate_plain, "p50_ms": float(np.percentile(lat, 50)),
                   "p99_ms": float(np.percentile(lat, 99))},
         "served_over_plain": rate_batch / rate_plain,

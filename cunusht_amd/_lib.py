@@ -41,6 +41,11 @@ class PlanDesc(ctypes.Structure):
 class PlanInfo(ctypes.Structure):
     _fields_ = [("ntheta", _i), ("ntheta_dfs", _i), ("nfine", _i), ("nrow_fine", _i), ("kernel_width", _i),
                 ("single_precision", _i), ("sigma", _d), ("beta", _d), ("workspace_bytes", _i64),
+                ("legendre_pair_steps", _i64), ("legendre_variant", _i), ("legendre_ms", ctypes.c_float)]
+
+
+class RingInfo(ctypes.Structure):
+    _fields_ = [("npix", _i64), ("nring", _i), ("ring_pairs", _i), ("ring_dft", _i), ("legendre_pair_steps", _i64),
                 ("legendre_variant", _i), ("legendre_ms", ctypes.c_float)]
 
 
@@ -58,6 +63,7 @@ SIGNATURES = {
     "nusht_ringplan_create": [ctypes.POINTER(RingDesc), _vp, ctypes.POINTER(_vp)],
     "nusht_ringplan_destroy": [_vp],
     "nusht_ringplan_npix": [_vp],
+    "nusht_ringplan_get_info": [_vp, ctypes.POINTER(RingInfo)],
     "nusht_ringplan_synthesis": [_vp, _vp, _i, _vp, _vp, _vp],
     "nusht_ringplan_adjoint": [_vp, _vp, _vp, _vp, _i, _vp],
     "nusht_ringplan_dlm2pointing": [_vp, _vp, _vp, _vp, _vp, _vp],

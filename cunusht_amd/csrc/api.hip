@@ -112,6 +112,21 @@ int nusht_ringplan_destroy(nusht_ringplan_t p) {
 
 int64_t nusht_ringplan_npix(nusht_ringplan_t p) { return p ? p->rp->npix() : -1; }
 
+int nusht_ringplan_get_info(nusht_ringplan_t p, nusht_ringplan_info* info) {
+  return guard([&] {
+    NUSHT_REQUIRE(p, kErrArg, "null ring plan");
+    NUSHT_REQUIRE(info, kErrArg, "null info");
+    const LegendreTables& t = p->rp->tables();
+    info->npix = p->rp->npix();
+    info->nring = p->rp->nring();
+    info->ring_pairs = t.nslot;
+    info->ring_dft = (int)p->rp->method();
+    info->legendre_pair_steps = t.pair_steps;
+    info->legendre_variant = t.variant;
+    info->legendre_ms = t.tuned_ms;
+  });
+}
+
 int nusht_ringplan_synthesis(nusht_ringplan_t p, const double* alm, int ncomp, double* map0, double* map1,
                              nusht_stream_t s) {
   return guard([&] {
@@ -183,6 +198,7 @@ int nusht_plan_get_info(nusht_plan_t p, nusht_plan_info* info) {
     info->sigma = p->np.sigma;
     info->beta = p->np.beta;
     info->workspace_bytes = (int64_t)(p->tab.bytes() + p->nu->bytes() + p->coef.bytes() + p->G.bytes() + p->ptg.bytes());
+    info->legendre_pair_steps = p->tab.pair_steps;
     info->legendre_variant = p->tab.variant;
     info->legendre_ms = p->tab.tuned_ms;
   });

@@ -39,6 +39,7 @@ struct LegendreTables {
   DevBuf st_k;            // double2[(lmax+1) * sstride]  scale exponents (k_u, k_v) at l = start
   DevBuf start;           // int[(lmax+1) * ngroups]
   long nlm = 0;           // sum over m of (lmax - l0(m) + 1)
+  long pair_steps = 0;    // (l, ring pair) recursion steps one synthesis runs (from `start`)
   std::string tag = "rings";  // stage-timer label ("cc" for the nuSHT grid, "ring" for the pointing grid)
   std::string n_prep, n_synth, n_adj;
   int chunk = 2;          // maps per synthesis launch

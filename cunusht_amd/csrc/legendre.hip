@@ -686,6 +686,16 @@ void LegendreTables::build(int L, int s, const std::vector<double>& theta, hipSt
                        slot_theta.as<double>(), mofs.as<long>(), rec.as<double2>(), st_mu.as<double2>(),
                        st_p.as<double2>(), st_k.as<double2>(), start.as<int>());
   NUSHT_LAUNCH_CHECK();
+  // work actually run by a synthesis (reported with the plan; synchronizes once)
+  std::vector<int> hs((size_t)(L + 1) * ngroups);
+  NUSHT_HIP(hipMemcpyAsync(hs.data(), start.get(), sizeof(int) * hs.size(), hipMemcpyDeviceToHost, st));
+  NUSHT_HIP(hipStreamSynchronize(st));
+  pair_steps = 0;
+  for (int m = 0; m <= L; ++m) {
+    if (l0_of(m, s) > L) continue;
+    for (int g = 0; g < ngroups; ++g)
+      pair_steps += (long)(L + 1 - hs[(size_t)m * ngroups + g]) * std::min(64, nslot - 64 * g);
+  }
 }
 
 static int padded_maps(int nmaps) { return nmaps <= 1 ? 1 : nmaps <= 2 ? 2 : nmaps <= 4 ? 4 : 8; }

@@ -36,6 +36,8 @@ struct NufftParams {
   int row_lo = 0;        // first fine theta row kept (may be negative: wraps)
   int nrow = 0;          // number of fine theta rows kept
   int nm = 0;            // 2 lmax + 1 phi modes
+  int poly_deg = 0;      // degree of the piecewise-polynomial kernel (interior cells)
+  double poly_err = 0;   // its measured max abs error against the exact kernel
 
   static NufftParams choose(int lmax, int spin, double eps, double sigma_hint = 0.0);
 };
@@ -73,6 +75,7 @@ class NufftPlan {
   int ntile_r_ = 0, ntile_c_ = 0;
   // tables
   DevBuf deconv_;                  // double[lmax+1] : 1 / psi_hat(k)
+  DevBuf kpoly_;                   // T[(poly_deg+1) * (w-2)] kernel polynomials, degree-major
   FftLength<float> ftd_f_, fnf_f_;
   FftLength<double> ftd_d_, fnf_d_;
   // buffers

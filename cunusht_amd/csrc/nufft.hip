@@ -232,6 +232,8 @@ template <typename T>
 struct InterpArgs {
   int nf, w, row_lo, nrow, spin, tr, tc, ntc;
   T beta;
+  const T* kpoly;  // [(deg + 1)][w - 2] interior-cell kernel polynomials
+  int deg;
   long npts;
   const double2* pts;
   const int* idx;
@@ -242,8 +244,109 @@ struct InterpArgs {
   double* out_im;
 };
 
-// One workgroup per fine-grid tile: the tile plus its (w-1)-cell halo is staged in LDS once and
-// every point binned to the tile is interpolated from it.
+// Kernel weights of one point along both axes for a compile-time width W:
+// k[i] = psi((f + i) 2/W) with f = (first cell) - (position) in [-W/2, -W/2 + 1).  Interior
+// cells come from the plan's Horner polynomials in u = f + W/2 - 1/2 (coefficients are
+// wave-uniform), the two outer cells from the exact kernel.
+template <typename T, int W>
+__device__ __forceinline__ void ker_weights(double ft, double fp, T beta, const T* __restrict__ cp, int deg,
+                                            T (&kt)[W], T (&kp)[W]) {
+  const T inv = T(2) / T(W);
+  kt[0] = WorkT<T>::ker((T)ft * inv, beta);
+  kp[0] = WorkT<T>::ker((T)fp * inv, beta);
+  if constexpr (W > 1) {
+    kt[W - 1] = WorkT<T>::ker((T)(ft + (W - 1)) * inv, beta);
+    kp[W - 1] = WorkT<T>::ker((T)(fp + (W - 1)) * inv, beta);
+  }
+  if constexpr (W > 2) {
+    constexpr int NI = W - 2;
+    const T ut = (T)(ft + 0.5 * W - 0.5), up = (T)(fp + 0.5 * W - 0.5);
+    const T* c = cp + (long)deg * NI;
+#pragma unroll
+    for (int i = 0; i < NI; ++i) { kt[i + 1] = c[i]; kp[i + 1] = c[i]; }
+    for (int k = deg - 1; k >= 0; --k) {
+      c -= NI;
+#pragma unroll
+      for (int i = 0; i < NI; ++i) {
+        kt[i + 1] = fma(kt[i + 1], ut, c[i]);
+        kp[i + 1] = fma(kp[i + 1], up, c[i]);
+      }
+    }
+  }
+}
+
+// One workgroup per fine-grid tile: the tile plus its (W-1)-cell halo is staged in LDS once and
+// every point binned to the tile is interpolated from it (fully unrolled W x W footprint).
+template <typename T, int W>
+__global__ __launch_bounds__(256) void k_interp_w(InterpArgs<T> a) {
+  using T2 = c2_t<T>;
+  extern __shared__ unsigned char smem_raw[];
+  T2* tile = reinterpret_cast<T2*>(smem_raw);
+  const int tb = blockIdx.x;
+  const int s0 = a.tile_start[tb], s1 = a.tile_start[tb + 1];
+  if (s0 == s1) return;
+  const int tw = a.tc + W, th = a.tr + W;
+  const int tile_r = tb / a.ntc, tile_c = tb - (tb / a.ntc) * a.ntc;
+  const int r0 = tile_r * a.tr, c0 = tile_c * a.tc;
+  for (int i = threadIdx.x; i < tw * th; i += blockDim.x) {
+    const int rr = i / tw, cc = i - rr * tw;
+    const int grow = r0 + rr;
+    int gc = c0 + cc;
+    while (gc >= a.nf) gc -= a.nf;
+    tile[i] = grow < a.nrow ? a.grid[(long)grow * a.nf + gc] : T2{T(0), T(0)};
+  }
+  __syncthreads();
+  const double scale = a.nf / (2.0 * M_PI);
+  const double hw = 0.5 * W;
+  for (int p = s0 + threadIdx.x; p < s1; p += blockDim.x) {
+    const double2 tp = a.pts[p];
+    const double ut = tp.x * scale;
+    double up = tp.y * scale;
+    up -= a.nf * floor(up / a.nf);
+    const int i0 = (int)ceil(ut - hw);
+    const int j0u = (int)ceil(up - hw);
+    const int j0 = j0u < 0 ? j0u + a.nf : j0u;
+    const int lr = i0 - a.row_lo - r0, lc = j0 - c0;
+    const int o = a.idx[p];
+    const bool ok = lr >= 0 && lr < a.tr && lc >= 0 && lc < a.tc;  // always true for binned points
+    T kt[W], kp[W];
+    ker_weights<T, W>(i0 - ut, j0u - up, a.beta, a.kpoly, a.deg, kt, kp);
+    T sr = 0, si = 0;
+    if (ok) {
+      const T2* base = tile + lr * tw + lc;
+#pragma unroll
+      for (int i = 0; i < W; ++i) {
+        const T2* row = base + i * tw;
+        T rr = 0, ri = 0;
+#pragma unroll
+        for (int j = 0; j < W; ++j) {
+          const T2 g = row[j];
+          rr = fma(g.x, kp[j], rr);
+          ri = fma(g.y, kp[j], ri);
+        }
+        sr = fma(rr, kt[i], sr);
+        si = fma(ri, kt[i], si);
+      }
+    }
+    double vr = ok ? (double)sr : __builtin_nan("");
+    double vi = ok ? (double)si : __builtin_nan("");
+    if (a.spin == 0) {
+      a.out_re[o] = vr;
+    } else {
+      if (a.gamma) {
+        double sn, cs;
+        sincos(a.spin * a.gamma[o], &sn, &cs);
+        const double xr = vr * cs - vi * sn, xi = vr * sn + vi * cs;
+        vr = xr;
+        vi = xi;
+      }
+      a.out_re[o] = vr;
+      a.out_im[o] = vi;
+    }
+  }
+}
+
+// Same for any width (runtime w, exact kernel evaluation); used above the templated widths.
 template <typename T>
 __global__ __launch_bounds__(256) void k_interp(InterpArgs<T> a) {
   using T2 = c2_t<T>;
@@ -324,6 +427,8 @@ template <typename T>
 struct SpreadArgs {
   int nf, w, row_lo, nrow, spin, tr, tc, ntc;
   T beta;
+  const T* kpoly;
+  int deg;
   const double2* pts;
   const int* idx;
   const int* tile_start;
@@ -332,6 +437,73 @@ struct SpreadArgs {
   const double* gamma;
   c2_t<T>* grid;
 };
+
+// Exact transpose of k_interp_w: points spread into an LDS copy of the tile (+ halo) with LDS
+// atomics, then the tile is added to the fine grid with global atomics.
+template <typename T, int W>
+__global__ __launch_bounds__(256) void k_spread_w(SpreadArgs<T> a) {
+  using T2 = c2_t<T>;
+  extern __shared__ unsigned char smem_raw[];
+  T* tile = reinterpret_cast<T*>(smem_raw);  // interleaved re, im
+  const int tb = blockIdx.x;
+  const int s0 = a.tile_start[tb], s1 = a.tile_start[tb + 1];
+  if (s0 == s1) return;
+  const int tw = a.tc + W, th = a.tr + W;
+  const int ncell = tw * th;
+  for (int i = threadIdx.x; i < 2 * ncell; i += blockDim.x) tile[i] = T(0);
+  __syncthreads();
+  const int tile_r = tb / a.ntc, tile_c = tb % a.ntc;
+  const int r0 = tile_r * a.tr, c0 = tile_c * a.tc;
+  const double scale = a.nf / (2.0 * M_PI);
+  const double hw = 0.5 * W;
+  for (int p = s0 + threadIdx.x; p < s1; p += blockDim.x) {
+    const double2 tp = a.pts[p];
+    const double ut = tp.x * scale;
+    double up = tp.y * scale;
+    up -= a.nf * floor(up / a.nf);
+    const int i0 = (int)ceil(ut - hw);
+    const int j0u = (int)ceil(up - hw);
+    const int j0 = j0u < 0 ? j0u + a.nf : j0u;
+    const int lr = i0 - a.row_lo - r0, lc = j0 - c0;
+    if (lr < 0 || lr >= a.tr || lc < 0 || lc >= a.tc) continue;  // never for points binned by set_pts
+    const int o = a.idx[p];
+    double vr = a.in_re[o], vi = (a.spin && a.in_im) ? a.in_im[o] : 0.0;
+    if (a.spin && a.gamma) {
+      double sn, cs;
+      sincos(-a.spin * a.gamma[o], &sn, &cs);
+      double xr = vr * cs - vi * sn, xi = vr * sn + vi * cs;
+      vr = xr; vi = xi;
+    }
+    T kt[W], kp[W];
+    ker_weights<T, W>(i0 - ut, j0u - up, a.beta, a.kpoly, a.deg, kt, kp);
+    T* base = tile + 2 * (lr * tw + lc);
+#pragma unroll
+    for (int i = 0; i < W; ++i) {
+      const T cr = (T)vr * kt[i], ci = (T)vi * kt[i];
+      T* trow = base + 2 * i * tw;
+#pragma unroll
+      for (int j = 0; j < W; ++j) {
+        atomicAdd(&trow[2 * j], cr * kp[j]);
+        atomicAdd(&trow[2 * j + 1], ci * kp[j]);
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < ncell; i += blockDim.x) {
+    const int rr = i / tw, cc = i - rr * tw;
+    const int grow = r0 + rr;
+    if (grow >= a.nrow) continue;
+    T vr = tile[2 * i], vi = tile[2 * i + 1];
+    if (vr == T(0) && vi == T(0)) continue;
+    const int gc = (c0 + cc) % a.nf;  // a tile can be wider than a small grid
+    T* g = reinterpret_cast<T*>(a.grid + (long)grow * a.nf + gc);
+    atomicAdd(&g[0], vr);
+    atomicAdd(&g[1], vi);
+  }
+}
+
+// Widths with a fully unrolled kernel; wider kernels use the runtime-width versions.
+constexpr int kMaxUnrolledW = 16;
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_spread(SpreadArgs<T> a) {
@@ -521,6 +693,26 @@ void launch_pass(K kern, int nblocks, size_t lds, const A& args, hipStream_t st)
 }  // namespace
 
 // ------------------------------------------------------------------ parameters
+template <typename T, int W>
+void launch_interp(int w, dim3 grid, size_t lds, hipStream_t st, const InterpArgs<T>& a) {
+  if constexpr (W > kMaxUnrolledW) {
+    hipLaunchKernelGGL(k_interp<T>, grid, dim3(256), lds, st, a);
+  } else {
+    if (w == W) hipLaunchKernelGGL((k_interp_w<T, W>), grid, dim3(256), lds, st, a);
+    else launch_interp<T, W + 1>(w, grid, lds, st, a);
+  }
+}
+
+template <typename T, int W>
+void launch_spread(int w, dim3 grid, size_t lds, hipStream_t st, const SpreadArgs<T>& a) {
+  if constexpr (W > kMaxUnrolledW) {
+    hipLaunchKernelGGL(k_spread<T>, grid, dim3(256), lds, st, a);
+  } else {
+    if (w == W) hipLaunchKernelGGL((k_spread_w<T, W>), grid, dim3(256), lds, st, a);
+    else launch_spread<T, W + 1>(w, grid, lds, st, a);
+  }
+}
+
 NufftParams NufftParams::choose(int lmax, int spin, double eps, double sigma_hint) {
   NUSHT_REQUIRE(lmax >= 1, kErrArg, "nufft: lmax must be >= 1");
   NUSHT_REQUIRE(eps > 0 && eps < 1, kErrArg, "nufft: epsilon must be in (0, 1)");
@@ -561,6 +753,75 @@ NufftParams NufftParams::choose(int lmax, int spin, double eps, double sigma_hin
 }
 
 // ------------------------------------------------------------------ plan
+// Piecewise-polynomial ES kernel.  A point at fractional offset t in [0, 1) from its first cell
+// needs psi(z_i), z_i = (t + i - w/2) 2/w, i = 0..w-1.  For the interior cells i = 1..w-2 the
+// kernel is analytic in t and psi(z_i) = sum_k c[k][i-1] u^k, u = t - 1/2, is interpolated at
+// degree + 1 Chebyshev nodes; the two outer cells hold the square-root branch point at |z| = 1
+// and are evaluated exactly.  The degree is the lowest in [2, 24] whose max error on a dense
+// sample (evaluated by Horner in the working precision) is below `target`.
+static void fit_kernel_poly(int w, double beta, double target, bool single, int& deg, double& err,
+                            std::vector<double>& coef) {
+  const int ni = w - 2;
+  deg = 0; err = 0; coef.clear();
+  if (ni <= 0) return;
+  auto psi = [&](long double z) -> long double {
+    long double t = 1.0L - z * z;
+    return t > 0 ? std::exp((long double)beta * (std::sqrt(t) - 1.0L)) : 0.0L;
+  };
+  double best_err = 1e300;
+  for (int d = 2; d <= 24; ++d) {
+    const int n = d + 1;
+    std::vector<double> c((size_t)n * ni);
+    std::vector<long double> un(n);
+    for (int j = 0; j < n; ++j) un[j] = 0.5L * std::cos((long double)M_PI * (j + 0.5L) / n);
+    for (int i = 1; i <= ni; ++i) {
+      // Vandermonde solve in long double (Gaussian elimination, partial pivoting)
+      std::vector<long double> A((size_t)n * (n + 1));
+      for (int r = 0; r < n; ++r) {
+        long double pw = 1.0L;
+        for (int k = 0; k < n; ++k) { A[(size_t)r * (n + 1) + k] = pw; pw *= un[r]; }
+        A[(size_t)r * (n + 1) + n] = psi((un[r] + 0.5L + i - 0.5L * w) * 2.0L / w);
+      }
+      for (int col = 0; col < n; ++col) {
+        int piv = col;
+        for (int r = col + 1; r < n; ++r)
+          if (std::fabs(A[(size_t)r * (n + 1) + col]) > std::fabs(A[(size_t)piv * (n + 1) + col])) piv = r;
+        if (piv != col)
+          for (int k = 0; k <= n; ++k) std::swap(A[(size_t)col * (n + 1) + k], A[(size_t)piv * (n + 1) + k]);
+        for (int r = col + 1; r < n; ++r) {
+          const long double f = A[(size_t)r * (n + 1) + col] / A[(size_t)col * (n + 1) + col];
+          for (int k = col; k <= n; ++k) A[(size_t)r * (n + 1) + k] -= f * A[(size_t)col * (n + 1) + k];
+        }
+      }
+      std::vector<long double> x(n);
+      for (int r = n - 1; r >= 0; --r) {
+        long double v = A[(size_t)r * (n + 1) + n];
+        for (int k = r + 1; k < n; ++k) v -= A[(size_t)r * (n + 1) + k] * x[k];
+        x[r] = v / A[(size_t)r * (n + 1) + r];
+      }
+      for (int k = 0; k < n; ++k) c[(size_t)k * ni + (i - 1)] = (double)x[k];
+    }
+    double e = 0;
+    for (int i = 1; i <= ni; ++i)
+      for (int q = 0; q <= 512; ++q) {
+        const double u = -0.5 + q / 512.0;
+        double v;
+        if (single) {
+          float fv = (float)c[(size_t)d * ni + (i - 1)];
+          for (int k = d - 1; k >= 0; --k) fv = std::fma(fv, (float)u, (float)c[(size_t)k * ni + (i - 1)]);
+          v = fv;
+        } else {
+          v = c[(size_t)d * ni + (i - 1)];
+          for (int k = d - 1; k >= 0; --k) v = std::fma(v, u, c[(size_t)k * ni + (i - 1)]);
+        }
+        e = std::max(e, (double)std::fabs((long double)v - psi((u + 0.5L + i - 0.5L * w) * 2.0L / w)));
+      }
+    if (e < best_err) { best_err = e; deg = d; coef = c; }
+    if (e <= target) break;
+  }
+  err = best_err;
+}
+
 NufftPlan::NufftPlan(const NufftParams& p, long npts_max, hipStream_t st) : p_(p), npts_max_(npts_max) {
   // deconvolution table: 1 / int_{-w/2}^{w/2} psi(2t/w) cos(2 pi k t / nf) dt
   std::vector<double> xq, wq;
@@ -577,6 +838,18 @@ NufftPlan::NufftPlan(const NufftParams& p, long npts_max, hipStream_t st) : p_(p
     dec[k] = 1.0 / (0.5 * p.w * s);
   }
   upload(deconv_, dec);
+  {
+    std::vector<double> c;
+    const double target = std::max(p.eps / 100.0, p.single ? 3e-7 : 1e-15);
+    fit_kernel_poly(p.w, p.beta, target, p.single, p_.poly_deg, p_.poly_err, c);
+    if (c.empty()) c.push_back(0.0);
+    if (p.single) {
+      std::vector<float> cf(c.begin(), c.end());
+      upload(kpoly_, cf);
+    } else {
+      upload(kpoly_, c);
+    }
+  }
   if (p.single) { ftd_f_.init(p.ntd); fnf_f_.init(p.nf); }
   else { ftd_d_.init(p.ntd); fnf_d_.init(p.nf); }
   const size_t cb = p.single ? 8 : 16;
@@ -599,7 +872,7 @@ NufftPlan::NufftPlan(const NufftParams& p, long npts_max, hipStream_t st) : p_(p
 }
 
 size_t NufftPlan::bytes() const {
-  return deconv_.bytes() + T_.bytes() + grid_.bytes() + pts_.bytes() + idx_.bytes() + key_.bytes() +
+  return deconv_.bytes() + kpoly_.bytes() + T_.bytes() + grid_.bytes() + pts_.bytes() + idx_.bytes() + key_.bytes() +
          tile_start_.bytes() + tile_cnt_.bytes() + cursor_.bytes() + scan_tmp_.bytes();
 }
 
@@ -678,11 +951,12 @@ void NufftPlan::type2_impl(const double2* G, long sm, const double* gamma, doubl
     NUSHT_LAUNCH_SIZED(k_phi2, T, p.nf, p.nrow, sizeof(T2) * (size_t)p.nf, pa, st);
   }
   if (npts_ == 0) return;
-  InterpArgs<T> ia{p.nf, p.w, p.row_lo, p.nrow, p.spin, tile_r_, tile_c_, ntile_c_, (T)p.beta, npts_,
-                   pts_.as<double2>(), idx_.as<int>(), tile_start_.as<int>(), grid_.as<T2>(), gamma, ore, oim};
+  InterpArgs<T> ia{p.nf, p.w, p.row_lo, p.nrow, p.spin, tile_r_, tile_c_, ntile_c_, (T)p.beta, kpoly_.as<T>(),
+                   p.poly_deg, npts_, pts_.as<double2>(), idx_.as<int>(), tile_start_.as<int>(), grid_.as<T2>(),
+                   gamma, ore, oim};
   StageScope sc("nufft_interp", st);
   const size_t tlds = sizeof(T2) * (size_t)(tile_r_ + p.w) * (tile_c_ + p.w);
-  hipLaunchKernelGGL(k_interp<T>, dim3(ntile_r_ * ntile_c_), dim3(256), tlds, st, ia);
+  launch_interp<T, 2>(p.w, dim3(ntile_r_ * ntile_c_), tlds, st, ia);
   NUSHT_LAUNCH_CHECK();
 }
 
@@ -702,10 +976,11 @@ void NufftPlan::type1_impl(const double* ire, const double* iim, const double* g
   }
   if (npts_ > 0) {
     StageScope sc1("nufft_spread", st);
-    SpreadArgs<T> sa{p.nf, p.w, p.row_lo, p.nrow, p.spin, tile_r_, tile_c_, ntile_c_, (T)p.beta, pts_.as<double2>(),
-                     idx_.as<int>(), tile_start_.as<int>(), ire, iim, gamma, grid_.as<T2>()};
+    SpreadArgs<T> sa{p.nf, p.w, p.row_lo, p.nrow, p.spin, tile_r_, tile_c_, ntile_c_, (T)p.beta, kpoly_.as<T>(),
+                     p.poly_deg, pts_.as<double2>(), idx_.as<int>(), tile_start_.as<int>(), ire, iim, gamma,
+                     grid_.as<T2>()};
     const size_t lds = sizeof(T2) * (size_t)(tile_r_ + p.w) * (tile_c_ + p.w);
-    hipLaunchKernelGGL(k_spread<T>, dim3(ntile_r_ * ntile_c_), dim3(256), lds, st, sa);
+    launch_spread<T, 2>(p.w, dim3(ntile_r_ * ntile_c_), lds, st, sa);
     NUSHT_LAUNCH_CHECK();
   }
   PhiArgs<T> pa{p.lmax, p.nf, p.nm, p.nrow, fnf, T_.as<T2>(), grid_.as<T2>()};

@@ -27,6 +27,7 @@ class RingPlan {
   long npix() const { return npix_; }
   int nring() const { return nring_; }
   RingMethod method() const { return method_; }
+  const LegendreTables& tables() const { return tab_; }
 
   // alm (ncomp x nalm complex) -> map0 (spin 0: real map; spin s: Q) and map1 (U)
   void synthesis(const double2* alm, int ncomp, double* map0, double* map1, hipStream_t st);

@@ -63,6 +63,11 @@ class RingSHT:
         _lib.call("nusht_ringplan_create", ctypes.byref(d), _stream(), ctypes.byref(h))
         self._h = h
 
+    def info(self) -> dict:
+        i = _lib.RingInfo()
+        _lib.call("nusht_ringplan_get_info", self._h, ctypes.byref(i))
+        return {f: getattr(i, f) for f, _ in _lib.RingInfo._fields_}
+
     def __del__(self):
         h = getattr(self, "_h", None)
         if h is not None and h.value:

@@ -84,6 +84,18 @@ typedef struct {
 int nusht_ringplan_create(const nusht_ring_desc* desc, nusht_stream_t s, nusht_ringplan_t* out);
 int nusht_ringplan_destroy(nusht_ringplan_t p);
 int64_t nusht_ringplan_npix(nusht_ringplan_t p);
+
+typedef struct {
+  int64_t npix;
+  int nring;
+  int ring_pairs;                /* rings after north/south pairing */
+  int ring_dft;                  /* 0 direct mixed-radix, 1 Rader, 2 Bluestein */
+  int64_t legendre_pair_steps;   /* (l, ring pair) recursion steps one synthesis runs */
+  int legendre_variant;          /* synthesis kernel variant chosen by the plan-time autotuner */
+  float legendre_ms;             /* its measured time for one map */
+} nusht_ringplan_info;
+/* Plan properties (no counterpart in the reference; reported for measurement). */
+int nusht_ringplan_get_info(nusht_ringplan_t p, nusht_ringplan_info* info);
 /* alm (ncomp x nalm) -> map0 (spin 0 map, or Q) and map1 (U; NULL for spin 0) */
 int nusht_ringplan_synthesis(nusht_ringplan_t p, const double* alm, int ncomp, double* map0, double* map1,
                              nusht_stream_t s);
@@ -118,6 +130,8 @@ typedef struct {
   double sigma;
   double beta;
   int64_t workspace_bytes;
+  int64_t legendre_pair_steps;  /* (l, ring pair) recursion steps one synthesis runs: l-ranges in
+                                   which a whole group of 64 ring pairs is below 2^-300 are skipped */
   int legendre_variant;  /* synthesis kernel variant chosen by the plan-time autotuner */
   float legendre_ms;     /* its measured time for one map */
 } nusht_plan_info;
