@@ -239,10 +239,15 @@ template <bool SP, int NB, int CH, int WPE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void k_synth(SynthArgs a) {
   constexpr int R = 1;
   constexpr int K = SP ? 2 * NB : NB;
+  // Blocks are dealt to the XCDs round-robin (block b runs on XCD b % 8).  XCD x takes
+  // m = x, x + 8, x + 16, ... with all slot blocks of one m in a row, so each m's coefficient
+  // stream is read through one XCD's L2, and every XCD still starts with its longest m.
   const int blk = blockIdx.x;
-  const int m = blk / a.nsb;
-  const int sb = blk - m * a.nsb;
+  const int j = blk / kNumXcd;
+  const int m = (blk - j * kNumXcd) + kNumXcd * (j / a.nsb);
+  const int sb = j % a.nsb;
   const int L = a.lmax;
+  if (m > L) return;
   const int l0 = l0_of(m, a.s);
   // the launch may carry one empty slot block per m (odd block count per m, see launch_synth)
   if (l0 > L || sb * 64 >= a.nslot) return;
@@ -277,20 +282,27 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       A0[r][b] = A1[r][b] = PN[r][b] = MN[r][b] = MS0[r][b] = MS1[r][b] = double2{0.0, 0.0};
     }
 
-  // One chunk of CH consecutive l (the first at even l - l0).  SLOW: some lane may still be
-  // below the representable range (masked contributions, rescaling); FULL: all CH l <= L.
-  auto chunk = [&](int lc, auto slow_tag, auto full_tag) {
-    constexpr bool SLOW = decltype(slow_tag)::value;
-    constexpr bool FULL = decltype(full_tag)::value;
-    const int j0 = lc - l0;
+  struct Buf {
     double2 rc[CH];
     double2 cw[CH][K];
+  };
+  // Wave-uniform coefficients of one chunk of CH consecutive l (scalar loads).  The loop below
+  // issues the loads of chunk c + 1 before it computes chunk c; the tables are padded by 2 CH
+  // entries for the prefetch past the last chunk.
+  auto load = [&](int lc, Buf& b) {
+    const int j0 = lc - l0;
 #pragma unroll
     for (int i = 0; i < CH; ++i) {
-      rc[i] = rec[j0 + i];
+      b.rc[i] = rec[j0 + i];
 #pragma unroll
-      for (int k = 0; k < K; ++k) cw[i][k] = cf[(long)(j0 + i) * K + k];
+      for (int k = 0; k < K; ++k) b.cw[i][k] = cf[(long)(j0 + i) * K + k];
     }
+  };
+  // One chunk (the first l at even l - l0).  SLOW: some lane may still be below the
+  // representable range (masked contributions, rescaling); FULL: all CH l <= L.
+  auto chunk = [&](int lc, const Buf& bf, auto slow_tag, auto full_tag) {
+    constexpr bool SLOW = decltype(slow_tag)::value;
+    constexpr bool FULL = decltype(full_tag)::value;
 #pragma unroll
     for (int i = 0; i < CH; ++i) {
       if (!FULL && lc + i > L) break;
@@ -302,9 +314,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
           if (!SP) {
-            if (odd) fma2(A1[r][b], cw[i][b], wu); else fma2(A0[r][b], cw[i][b], wu);
+            if (odd) fma2(A1[r][b], bf.cw[i][b], wu); else fma2(A0[r][b], bf.cw[i][b], wu);
           } else {
-            const double2 w = cw[i][2 * b], wt = cw[i][2 * b + 1];
+            const double2 w = bf.cw[i][2 * b], wt = bf.cw[i][2 * b + 1];
             fma2(PN[r][b], w, wu);
             fma2(MN[r][b], wt, wv);
             if (odd) { fma2(A1[r][b], w, wv); fma2(MS1[r][b], wt, wu); }
@@ -315,11 +327,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       // the step past l = lmax reads a padded table entry; its result is never used
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const double t = rc[i].x * x[r];
-        const double nu = fma(t + rc[i].y, mu_u[r], -pu[r]);
+        const double t = bf.rc[i].x * x[r];
+        const double nu = fma(t + bf.rc[i].y, mu_u[r], -pu[r]);
         pu[r] = mu_u[r]; mu_u[r] = nu;
         if (SP) {
-          const double nv = fma(t - rc[i].y, mu_v[r], -pv[r]);
+          const double nv = fma(t - bf.rc[i].y, mu_v[r], -pv[r]);
           pv[r] = mu_v[r]; mu_v[r] = nv;
         }
         if (SLOW) {
@@ -329,21 +341,33 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void 
       }
     }
   };
-  using T_ = std::true_type;
-  using F_ = std::false_type;
-  int lc = lstart;
-  // slow phase: while some lane of the wave is still below 2^-300
-  for (; lc <= L; lc += CH) {
+  auto any_under = [&]() {
     bool under = false;
 #pragma unroll
     for (int r = 0; r < R; ++r) under |= (ku[r] < 0.0) || (kv[r] < 0.0);
-    if (!__any(under)) break;
-    if (lc + CH - 1 <= L) chunk(lc, T_{}, T_{});
-    else chunk(lc, T_{}, F_{});
+    return __any(under);
+  };
+  using T_ = std::true_type;
+  using F_ = std::false_type;
+  int lc = lstart;
+  Buf cur, nxt;
+  load(lc, cur);
+  // slow phase: while some lane of the wave is still below 2^-300
+  while (lc <= L && any_under()) {
+    load(lc + CH, nxt);
+    if (lc + CH - 1 <= L) chunk(lc, cur, T_{}, T_{});
+    else chunk(lc, cur, T_{}, F_{});
+    cur = nxt;
+    lc += CH;
   }
   // fast phase
-  for (; lc + CH - 1 <= L; lc += CH) chunk(lc, F_{}, T_{});
-  if (lc <= L) chunk(lc, F_{}, F_{});
+  while (lc + CH - 1 <= L) {
+    load(lc + CH, nxt);
+    chunk(lc, cur, F_{}, T_{});
+    cur = nxt;
+    lc += CH;
+  }
+  if (lc <= L) chunk(lc, cur, F_{}, F_{});
 
   const double sg0 = ((l0 + m) & 1) ? -1.0 : 1.0;  // (-1)^{l+m} for even (l - l0)
 #pragma unroll
@@ -660,9 +684,9 @@ void LegendreTables::build(int L, int s, const std::vector<double>& theta, hipSt
   }
   nlm = mo[L + 1];
   upload(mofs, mo);
-  // padded so that a chunked l-loop may read up to kSynCH entries past the last one
-  rec.alloc(sizeof(double2) * (size_t)(std::max<long>(nlm, 1) + kSynCH));
-  cnorm.alloc(sizeof(double) * (size_t)(std::max<long>(nlm, 1) + kSynCH));
+  // padded so that a chunked, prefetching l-loop may read up to 2 kSynCH entries past the last one
+  rec.alloc(sizeof(double2) * (size_t)(std::max<long>(nlm, 1) + 2 * kSynCH));
+  cnorm.alloc(sizeof(double) * (size_t)(std::max<long>(nlm, 1) + 2 * kSynCH));
   sstride = nslot;
   ngroups = ceil_div(nslot, 64);
   const size_t nst = (size_t)(L + 1) * sstride;
@@ -701,8 +725,8 @@ void LegendreTables::build(int L, int s, const std::vector<double>& theta, hipSt
 static int padded_maps(int nmaps) { return nmaps <= 1 ? 1 : nmaps <= 2 ? 2 : nmaps <= 4 ? 4 : 8; }
 
 size_t legendre_coef_bytes(const LegendreTables& t, int nmaps) {
-  // padded by kSynCH entries (chunked reads past the last l of the last m)
-  return sizeof(double2) * (size_t)(std::max<long>(t.nlm, 1) + kSynCH) * padded_maps(nmaps) * (t.spin ? 2 : 1);
+  // padded by 2 kSynCH entries (chunked, prefetching reads past the last l of the last m)
+  return sizeof(double2) * (size_t)(std::max<long>(t.nlm, 1) + 2 * kSynCH) * padded_maps(nmaps) * (t.spin ? 2 : 1);
 }
 
 // Synthesis variants (l-chunk CH, minimum waves per SIMD); the autotuner picks one per table.
@@ -718,7 +742,7 @@ static void launch_synth(const SynthArgs& a, int variant, hipStream_t st) {
   // with a power-of-two count every engine would see the same slot blocks for every m, so the
   // polar blocks (long underflow phases) would pile up on a few engines.
   const int nsb = ceil_div(a.nslot, 64) | 1;
-  const int nblocks = nsb * (a.lmax + 1);
+  const int nblocks = nsb * ceil_div(a.lmax + 1, kNumXcd) * kNumXcd;
   SynthArgs b = a;
   b.nsb = nsb;
   switch (variant) {
