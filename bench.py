@@ -176,7 +176,7 @@ def main():
     def batched_step(i):
         d = dlms[i % nin]
         _lib.call("nusht_ringplan_dlm2pointing", ring._h, P(d), P(ptg[0]), P(ptg[1]), P(ptg[2]), stream())
-        _lib.call("nusht_plan_set_pts", plan._h, P(ptg[0]), P(ptg[1]), npix, stream())
+        _lib.call("nusht_plan_set_pts_async", plan._h, P(ptg[0]), P(ptg[1]), npix, stream())
         plan.pts_key = None
         done = 0
         while done < B:

@@ -71,6 +71,7 @@ SIGNATURES = {
     "nusht_plan_destroy": [_vp],
     "nusht_plan_get_info": [_vp, ctypes.POINTER(PlanInfo)],
     "nusht_plan_set_pts": [_vp, _vp, _vp, _i64, _vp],
+    "nusht_plan_set_pts_async": [_vp, _vp, _vp, _i64, _vp],
     "nusht_nusht2d2": [_vp, _vp, _vp, _vp, _vp, _vp],
     "nusht_nusht2d1": [_vp, _vp, _vp, _vp, _vp, _vp],
     "nusht_gclm2lenmap": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _vp],

@@ -211,6 +211,14 @@ int nusht_plan_set_pts(nusht_plan_t p, const double* theta, const double* phi, i
   });
 }
 
+int nusht_plan_set_pts_async(nusht_plan_t p, const double* theta, const double* phi, int64_t npts,
+                             nusht_stream_t s) {
+  return guard([&] {
+    check_plan(p);
+    p->nu->set_pts(theta, phi, npts, S(s), /*check=*/false);
+  });
+}
+
 static void synth_batch(nusht_plan_t p, int nmaps, const double* alm, long alm_stride, double* out0, double* out1,
                         long out_stride, const double* gamma, hipStream_t st) {
   const long gmap = (long)p->nmrows * p->np.ntheta;
@@ -266,7 +274,7 @@ int nusht_gclm2lenmap(nusht_plan_t p, nusht_ringplan_t ptg, const double* gclm, 
     double* ga = ph + p->npts_max;
     const bool rot = polrot && p->spin > 0;
     ptg->rp->dlm2pointing(reinterpret_cast<const double2*>(dlm_scaled), th, ph, rot ? ga : nullptr, S(s));
-    p->nu->set_pts(th, ph, npix, S(s));
+    p->nu->set_pts(th, ph, npix, S(s), /*check=*/false);  // pointing from the ring plan
     synth_batch(p, 1, gclm, 0, lenmap0, lenmap1, 0, rot ? ga : nullptr, S(s));
   });
 }
@@ -284,7 +292,7 @@ int nusht_lenmap2gclm(nusht_plan_t p, nusht_ringplan_t ptg, const double* lenmap
     double* ga = ph + p->npts_max;
     const bool rot = polrot && p->spin > 0;
     ptg->rp->dlm2pointing(reinterpret_cast<const double2*>(dlm_scaled), th, ph, rot ? ga : nullptr, S(s));
-    p->nu->set_pts(th, ph, npix, S(s));
+    p->nu->set_pts(th, ph, npix, S(s), /*check=*/false);  // pointing from the ring plan
     p->nu->type1(lenmap0, lenmap1, rot ? ga : nullptr, p->G.as<double2>(), p->np.ntheta, S(s));
     legendre_adjoint(p->tab, p->ncomp, p->G.as<double2>(), p->np.ntheta, 1, reinterpret_cast<double2*>(gclm), S(s));
   });

@@ -49,7 +49,11 @@ class NufftPlan {
   const NufftParams& params() const { return p_; }
 
   // Sort points into fine-grid tiles (theta, phi in radians; theta in [0, pi]).
-  void set_pts(const double* theta, const double* phi, long npts, hipStream_t st);
+  // Bins the points by fine-grid tile.  With `check`, synchronizes once and rejects points with
+  // theta outside the fine grid's rows (or NaN); without it (points produced on the device by
+  // this library), such points are binned to a tile whose footprint does not contain them, so
+  // interpolation returns NaN for them and spreading skips them.
+  void set_pts(const double* theta, const double* phi, long npts, hipStream_t st, bool check = true);
   long npts() const { return npts_; }
 
   // type 2: G (rows m, ntheta rings; row of m = (spin ? lmax + m : m)) -> values.

@@ -91,8 +91,11 @@ class NuPlan:
         _lib.call("nusht_plan_get_info", self._h, ctypes.byref(i))
         return {f: getattr(i, f) for f, _ in _lib.PlanInfo._fields_}
 
-    def set_pts(self, theta: torch.Tensor, phi: torch.Tensor):
-        _lib.call("nusht_plan_set_pts", self._h, _ptr(theta), _ptr(phi), theta.numel(), _stream())
+    def set_pts(self, theta: torch.Tensor, phi: torch.Tensor, checked: bool = True):
+        """checked: bounds-check the points (one stream synchronization, raises on a bad point);
+        otherwise bad points give NaN in nusht2d2 and are ignored by nusht2d1."""
+        fn = "nusht_plan_set_pts" if checked else "nusht_plan_set_pts_async"
+        _lib.call(fn, self._h, _ptr(theta), _ptr(phi), theta.numel(), _stream())
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -168,7 +171,8 @@ class GPU_nusht_transformer:
     def _ensure_pts(self, plan: NuPlan):
         theta, phi, key = self._cur_pts
         if plan.pts_key != key:
-            plan.set_pts(theta, phi)
+            # pointing computed here from dlm_scaled lies in [0, pi] by construction: no host check
+            plan.set_pts(theta, phi, checked=key[0] != "dlm")
             plan.pts_key = key
 
     # ------------------------------------------------------------------ geometry helpers

@@ -143,6 +143,12 @@ int nusht_plan_get_info(nusht_plan_t p, nusht_plan_info* info);
 /* Points (theta in [0, pi], any phi).  Replaces set_pts / cuFINUFFT setpts
  * (GPU_nufft_transformer.py:249-251).  Synchronizes the stream once (bounds check). */
 int nusht_plan_set_pts(nusht_plan_t p, const double* theta, const double* phi, int64_t npts, nusht_stream_t s);
+/* Same without the bounds check and without synchronizing (the setpts of cuFINUFFT checks
+ * nothing either).  A point with theta outside [0, pi] beyond the kernel half-width, or NaN,
+ * gives NaN in nusht2d2 and is ignored by nusht2d1.  For points computed on the device, e.g. by
+ * nusht_ringplan_dlm2pointing. */
+int nusht_plan_set_pts_async(nusht_plan_t p, const double* theta, const double* phi, int64_t npts,
+                             nusht_stream_t s);
 
 /* alm -> values at the points.  Replaces nusht2d2 (GPU_nufft_transformer.py:386-435).
  * spin 0: out0[npts] real; spin s: out0 = Q, out1 = U, rotated by exp(i s gamma) if gamma != NULL. */
