@@ -235,23 +235,26 @@ struct SynthArgs {
 // entries so a chunk never reads past the end.
 constexpr int kSynCH = 8;  // largest chunk of any variant (table padding)
 
-template <bool SP, int NB, int CH, int WPE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WPE))) void k_synth(SynthArgs a) {
+template <bool SP, int NB, int CH, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_synth(SynthArgs a) {
   constexpr int R = 1;
   constexpr int K = SP ? 2 * NB : NB;
   // Blocks are dealt to the XCDs round-robin (block b runs on XCD b % 8).  XCD x takes
-  // m = x, x + 8, x + 16, ... with all slot blocks of one m in a row, so each m's coefficient
-  // stream is read through one XCD's L2, and every XCD still starts with its longest m.
+  // m = x, x + 8, x + 16, ... with all blocks of one m in a row, so each m's coefficient stream
+  // is read through one XCD's L2, and every XCD still starts with its longest m.  The WPB waves
+  // of a block take WPB adjacent slot groups of the same m and read the same coefficients, so
+  // all but the leading wave find them in the CU's scalar cache.
   const int blk = blockIdx.x;
   const int j = blk / kNumXcd;
   const int m = (blk - j * kNumXcd) + kNumXcd * (j / a.nsb);
-  const int sb = j % a.nsb;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int sb = (j % a.nsb) * WPB + wave;
   const int L = a.lmax;
   if (m > L) return;
   const int l0 = l0_of(m, a.s);
-  // the launch may carry one empty slot block per m (odd block count per m, see launch_synth)
+  // the launch may carry empty slot groups (odd block count per m, see launch_synth)
   if (l0 > L || sb * 64 >= a.nslot) return;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   const int lstart = a.start[(long)m * ((a.nslot + 63) >> 6) + sb];
 
   double x[R], mu_u[R], mu_v[R], pu[R], pv[R], ku[R], kv[R];
@@ -729,26 +732,29 @@ size_t legendre_coef_bytes(const LegendreTables& t, int nmaps) {
   return sizeof(double2) * (size_t)(std::max<long>(t.nlm, 1) + 2 * kSynCH) * padded_maps(nmaps) * (t.spin ? 2 : 1);
 }
 
-// Synthesis variants (l-chunk CH, minimum waves per SIMD); the autotuner picks one per table.
+// Synthesis variants (l-chunk CH, waves per block WPB); the autotuner picks one per table.
 struct SynVariant {
-  int CH, WPE;
+  int CH, WPB;
 };
-static const SynVariant kSynVariants[] = {{2, 1}, {4, 1}, {8, 1}};
-constexpr int kNumSynVariants = 3;
+static const SynVariant kSynVariants[] = {{2, 1}, {2, 4}, {4, 4}, {2, 8}};
+constexpr int kNumSynVariants = 4;
 
 template <bool SP, int NB>
 static void launch_synth(const SynthArgs& a, int variant, hipStream_t st) {
-  // Odd number of slot blocks per m: blocks are handed to XCDs and shader engines round-robin, and
-  // with a power-of-two count every engine would see the same slot blocks for every m, so the
-  // polar blocks (long underflow phases) would pile up on a few engines.
-  const int nsb = ceil_div(a.nslot, 64) | 1;
+  // Odd number of blocks per m: blocks are handed to XCDs and shader engines round-robin, and
+  // with a power-of-two count every engine would see the same slot groups for every m, so the
+  // polar groups (long underflow phases) would pile up on a few engines.
+  const int wpb = kSynVariants[variant].WPB;
+  const int nsb = ceil_div(ceil_div(a.nslot, 64), wpb) | 1;
   const int nblocks = nsb * ceil_div(a.lmax + 1, kNumXcd) * kNumXcd;
   SynthArgs b = a;
   b.nsb = nsb;
+  const dim3 blk(64 * wpb);
   switch (variant) {
-    case 0: hipLaunchKernelGGL((k_synth<SP, NB, 2, 1>), dim3(nblocks), dim3(64), 0, st, b); break;
-    case 1: hipLaunchKernelGGL((k_synth<SP, NB, 4, 1>), dim3(nblocks), dim3(64), 0, st, b); break;
-    default: hipLaunchKernelGGL((k_synth<SP, NB, 8, 1>), dim3(nblocks), dim3(64), 0, st, b); break;
+    case 0: hipLaunchKernelGGL((k_synth<SP, NB, 2, 1>), dim3(nblocks), blk, 0, st, b); break;
+    case 1: hipLaunchKernelGGL((k_synth<SP, NB, 2, 4>), dim3(nblocks), blk, 0, st, b); break;
+    case 2: hipLaunchKernelGGL((k_synth<SP, NB, 4, 4>), dim3(nblocks), blk, 0, st, b); break;
+    default: hipLaunchKernelGGL((k_synth<SP, NB, 2, 8>), dim3(nblocks), blk, 0, st, b); break;
   }
   NUSHT_LAUNCH_CHECK();
 }
@@ -832,8 +838,9 @@ void legendre_autotune(LegendreTables& t, int ncomp, double2* coef_ws, double2* 
     NUSHT_HIP(hipEventSynchronize(e1));
     float ms = 0;
     NUSHT_HIP(hipEventElapsedTime(&ms, e0, e1));
-    if (verbose) fprintf(stderr, "[nusht] legendre autotune %s lmax %d spin %d pairs %d: variant %d (CH %d) %.3f ms\n",
-                         t.tag.c_str(), t.lmax, t.spin, t.nslot, v, kSynVariants[v].CH, ms / 2);
+    if (verbose)
+      fprintf(stderr, "[nusht] legendre autotune %s lmax %d spin %d pairs %d: variant %d (CH %d, %d waves) %.3f ms\n",
+              t.tag.c_str(), t.lmax, t.spin, t.nslot, v, kSynVariants[v].CH, kSynVariants[v].WPB, ms / 2);
     if (ms < best) { best = ms; bestv = v; }
   }
   t.variant = bestv;
