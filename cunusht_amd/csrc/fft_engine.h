@@ -217,8 +217,14 @@ __device__ __forceinline__ void fft_stage(c2_t<T>* buf, int n, int ns, const c2_
 #pragma unroll
       for (int r = 0; r < R; ++r) v[i][r] = buf[j + r * stride];
       if (ns > 1) {
+        // one table read per butterfly; w^r by repeated multiplication (error ~ R ulp)
+        const T2 w1 = tw[k * twstep];
+        T2 wr = w1;
 #pragma unroll
-        for (int r = 1; r < R; ++r) v[i][r] = cmul(v[i][r], tw[k * r * twstep]);
+        for (int r = 1; r < R; ++r) {
+          v[i][r] = cmul(v[i][r], wr);
+          if (r + 1 < R) wr = cmul(wr, w1);
+        }
       }
       Dft<T, R>::run(v[i]);
     }

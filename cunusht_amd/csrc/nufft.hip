@@ -928,7 +928,7 @@ static constexpr int nmax_big() { return kLdsBytes / (int)sizeof(c2_t<T>); }
 #define NUSHT_LAUNCH_SIZED(KERN, T, n, nblocks, lds, args, st)                                        \
   do {                                                                                               \
     if ((n) <= 2048) launch_pass<T, 256, 2048>(KERN<T, 256, 2048>, nblocks, lds, args, st);           \
-    else launch_pass<T, 512, nmax_big<T>()>(KERN<T, 512, nmax_big<T>()>, nblocks, lds, args, st);     \
+    else launch_pass<T, 1024, nmax_big<T>()>(KERN<T, 1024, nmax_big<T>()>, nblocks, lds, args, st);     \
   } while (0)
 
 template <typename T>
