@@ -15,7 +15,7 @@ namespace nusht {
 namespace {
 
 // 640 threads: a radix-13 stage over 8190 points (Rader for nphi = 8191) needs one butterfly per thread.
-constexpr int kRingNT = 640;
+constexpr int kRingNT = 768;
 constexpr int kRingNMAX = 8320;
 
 struct RingArgs {
