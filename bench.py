@@ -52,20 +52,22 @@ def synth_inputs(lmax: int, rng: np.random.Generator):
 
 
 def stage_work(stage: str, info: dict, lmax: int, npix: int, maps_per_launch: int = 1, rinfo: dict | None = None):
-    """(flops or bytes, unit) of one launch of a stage.
+    """(flops or bytes, unit) of one launch of a stage: the algorithmic work per unit times the units
+    one launch processes.
 
-    Spin-s Legendre synthesis: per (l, m, ring pair) step that the kernel runs, 7 flops of
-    recursion (shared by the maps of one launch) plus 16 flops of contraction per map (4
-    complex-by-real FMAs: +m/-m, north/south).  The step count is the plan's
-    legendre_pair_steps: l-ranges in which a whole group of 64 ring pairs is below 2^-300 are
-    skipped, so they are not counted."""
+    Legendre synthesis: N_lm * ceil(n_rings / 2) (l, m, ring pair) units per map at 22 flops for a
+    spin-2 E+B map and 14 for the spin-1 gradient pair.  The kernels skip l-ranges in which a whole
+    group of 64 ring pairs is below 2^-300; `executed_steps` reports what they run.
+    nuFFT interpolation: N_pts (16 + 16) + nrow nf b_grid bytes (FP64 coordinates, complex FP64
+    values; the fine rows actually formed)."""
+    nlm = (lmax + 1) * (lmax + 2) // 2
     nt = info["ntheta"]
     single = info["single_precision"]
     bc = 8 if single else 16
     if stage == "legendre_synth_cc_s2":
-        return info["legendre_pair_steps"] * (7 + 16 * maps_per_launch), "flop"
+        return nlm * math.ceil(nt / 2) * 22 * maps_per_launch, "flop"
     if stage == "legendre_synth_ring_s1" and rinfo is not None:
-        return rinfo["legendre_pair_steps"] * (7 + 16), "flop"
+        return nlm * math.ceil(rinfo["nring"] / 2) * 14, "flop"
     if stage == "nufft_interp":
         return npix * (16 + 16) + info["nrow_fine"] * info["nfine"] * bc, "byte"
     if stage == "theta_pass_t2":
@@ -75,6 +77,30 @@ def stage_work(stage: str, info: dict, lmax: int, npix: int, maps_per_launch: in
     if stage == "ring_dft_pointing":
         return (lmax + 1) * (2 * lmax + 1) * 16 + npix * 24, "byte"
     return None, None
+
+
+def executed_flops(stage: str, info: dict, maps_per_launch: int = 1, rinfo: dict | None = None):
+    """Flops the Legendre kernels actually run: executed (l, ring pair) steps x (7 flops of
+    recursion + 16 of contraction per map)."""
+    if stage == "legendre_synth_cc_s2":
+        return info["legendre_pair_steps"] * (7 + 16 * maps_per_launch)
+    if stage == "legendre_synth_ring_s1" and rinfo is not None:
+        return rinfo["legendre_pair_steps"] * (7 + 16)
+    return None
+
+
+def profiled_traffic(stage: str, maps_per_launch: int):
+    """HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary
+    (profiles/traffic_spin2_lmax4095.md, copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_spin2_lmax4095.md")
+    if not os.path.exists(path):
+        return None
+    key = f"traffic_bytes_per_launch[{stage}, {maps_per_launch} maps] = "
+    with open(path) as f:
+        for line in f:
+            if line.strip().startswith(key):
+                return int(line.strip()[len(key):].split()[0])
+    return None
 
 
 def cpu_baseline(lmax_sample: int, lmax: int, eps: float):
@@ -109,8 +135,9 @@ def cpu_baseline(lmax_sample: int, lmax: int, eps: float):
     t_scaled = t_leg * f_leg + t_rest * f_rest
     return {"value": 1.0 / t_scaled, "unit": "lensed maps/s", "cores": len(os.sched_getaffinity(0)),
             "kind": "port",
-            "sample": f"oracle gclm2lenmap spin-2 lmax={L} eps={eps:g} (1 map, {t_leg + t_rest:.1f} s: "
-                      f"Legendre {t_leg:.1f} s scaled x{f_leg:.1f}, rest {t_rest:.1f} s scaled x{f_rest:.1f})"}
+            "sample": f"oracle gclm2lenmap spin-2 lmax={L} eps={eps:g}, 1 map on the GL {L + 1}x{2 * L + 1} grid "
+                      f"({time.perf_counter() - t0:.1f} s of wall time in all; per map: Legendre {t_leg:.1f} s "
+                      f"scaled x{f_leg:.1f} by N_lm * rings, rest {t_rest:.1f} s scaled x{f_rest:.1f} by pixels)"}
 
 
 def geometry_ntheta_ref(lmax: int) -> int:
@@ -127,7 +154,7 @@ def main():
     ap.add_argument("--eps", type=float, default=1e-7)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-lmax", type=int, default=1023)
+    ap.add_argument("--cpu-lmax", type=int, default=2559)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the timed region, copy a buffer of this many MiB once (a plain copy kernel "
                          "of known traffic, used to calibrate FETCH_SIZE / WRITE_SIZE counters)")
@@ -230,19 +257,26 @@ def main():
     name, (tot_ms, calls) = dom
     launches_cc = stages_batch.get("legendre_synth_cc_s2", (0, 1))[1]
     mpl = B * args.steps / max(launches_cc, 1) if name == "legendre_synth_cc_s2" else 1
+    mpl = int(round(mpl))
     work, unit = stage_work(name, info, L, npix, mpl, rinfo)
     roof = None
     if work is not None:
         avg_s = tot_ms / calls / 1e3
+        traffic = profiled_traffic(name, mpl)
         if unit == "flop":
             ach = work / avg_s / 1e12
             roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                    "frac": ach / PEAK_FP64_TFLOPS, "traffic": None, "kernel": name, "avg_ms": tot_ms / calls,
-                    "maps_per_launch": mpl}
+                    "frac": ach / PEAK_FP64_TFLOPS, "traffic": traffic, "kernel": name, "avg_ms": tot_ms / calls,
+                    "maps_per_launch": mpl, "algorithmic_flop": work}
+            ex = executed_flops(name, info, mpl, rinfo)
+            if ex is not None:
+                roof["executed_flop"] = ex
+                roof["executed_tflops"] = ex / avg_s / 1e12
+                roof["executed_frac"] = ex / avg_s / 1e12 / PEAK_FP64_TFLOPS
         else:
             ach = work / avg_s / 1e9
             roof = {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                    "frac": ach / PEAK_HBM_GBS, "traffic": None, "kernel": name, "avg_ms": tot_ms / calls}
+                    "frac": ach / PEAK_HBM_GBS, "traffic": traffic, "kernel": name, "avg_ms": tot_ms / calls}
     # the nuFFT interpolation against the HBM roofline
     interp = stages_batch.get("nufft_interp")
     nufft_roof = None
@@ -284,6 +318,7 @@ def main():
                    "nufft": {"nfine": info["nfine"], "nrow": info["nrow_fine"], "w": info["kernel_width"],
                              "sigma": round(info["sigma"], 4)},
                    "legendre_variant": info["legendre_variant"],
+                   "plan_bytes": info["workspace_bytes"],
                    "legendre_pair_steps": {"nusht_cc": info["legendre_pair_steps"],
                                            "full_cc": nlm * math.ceil(info["ntheta"] / 2),
                                            "pointing_gl": rinfo["legendre_pair_steps"],
