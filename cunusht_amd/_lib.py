@@ -64,6 +64,7 @@ SIGNATURES = {
     "nusht_ringplan_destroy": [_vp],
     "nusht_ringplan_npix": [_vp],
     "nusht_ringplan_get_info": [_vp, ctypes.POINTER(RingInfo)],
+    "nusht_legendre_variants": [],
     "nusht_ringplan_synthesis": [_vp, _vp, _i, _vp, _vp, _vp],
     "nusht_ringplan_adjoint": [_vp, _vp, _vp, _vp, _i, _vp],
     "nusht_ringplan_dlm2pointing": [_vp, _vp, _vp, _vp, _vp, _vp],

@@ -112,6 +112,8 @@ int nusht_ringplan_destroy(nusht_ringplan_t p) {
 
 int64_t nusht_ringplan_npix(nusht_ringplan_t p) { return p ? p->rp->npix() : -1; }
 
+int nusht_legendre_variants(void) { return legendre_variant_count(); }
+
 int nusht_ringplan_get_info(nusht_ringplan_t p, nusht_ringplan_info* info) {
   return guard([&] {
     NUSHT_REQUIRE(p, kErrArg, "null ring plan");

@@ -63,7 +63,11 @@ void legendre_synth(const LegendreTables& t, int nmaps, int ncomp, const double2
                     const double* fl, double2* coef_ws, double2* G, long g_stride_m, long g_stride_r,
                     long g_stride_map, hipStream_t st);
 
-// Times every synthesis variant on the given buffers (contents are overwritten) and keeps the fastest.
+// Number of synthesis kernel variants.
+int legendre_variant_count();
+
+// Times every synthesis variant on the given buffers (contents are overwritten) and keeps the fastest
+// (or takes the one named by the environment variable NUSHT_LEGENDRE_VARIANT).
 // (Plan creation only: allocates a scratch alm set and synchronizes.)
 void legendre_autotune(LegendreTables& t, int ncomp, double2* coef_ws, double2* G, long sm, long sr, hipStream_t st);
 

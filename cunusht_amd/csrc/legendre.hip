@@ -809,7 +809,17 @@ static void legendre_synth_chunk(const LegendreTables& t, int nmaps, int ncomp, 
   }
 }
 
+int legendre_variant_count() { return kNumSynVariants; }
+
 void legendre_autotune(LegendreTables& t, int ncomp, double2* coef_ws, double2* G, long sm, long sr, hipStream_t st) {
+  // NUSHT_LEGENDRE_VARIANT=k fixes the variant (tests run every variant this way)
+  if (const char* fv = getenv("NUSHT_LEGENDRE_VARIANT"); fv && fv[0]) {
+    const int v = atoi(fv);
+    NUSHT_REQUIRE(v >= 0 && v < kNumSynVariants, kErrArg, "NUSHT_LEGENDRE_VARIANT out of range");
+    t.variant = v;
+    t.tuned_ms = 0.f;
+    return;
+  }
   // one measurement per (lmax, spin, ring pairs) and process
   static std::mutex mu;
   static std::map<std::tuple<int, int, int, int>, std::pair<int, float>> cache;

@@ -96,6 +96,9 @@ typedef struct {
 } nusht_ringplan_info;
 /* Plan properties (no counterpart in the reference; reported for measurement). */
 int nusht_ringplan_get_info(nusht_ringplan_t p, nusht_ringplan_info* info);
+/* Number of Legendre synthesis kernel variants; plan creation times them and keeps the fastest unless the
+ * environment variable NUSHT_LEGENDRE_VARIANT names one (no counterpart in the reference). */
+int nusht_legendre_variants(void);
 /* alm (ncomp x nalm) -> map0 (spin 0 map, or Q) and map1 (U; NULL for spin 0) */
 int nusht_ringplan_synthesis(nusht_ringplan_t p, const double* alm, int ncomp, double* map0, double* map1,
                              nusht_stream_t s);

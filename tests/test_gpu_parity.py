@@ -181,6 +181,21 @@ def test_dlm2pointing_vs_oracle(L, nphi):
 @pytest.mark.parametrize("spin", [0, 1, 2])
 @pytest.mark.parametrize("mcut", [0, 40, 120])
 def test_ring_sht_polar_underflow(spin, mcut):
+    _polar_underflow_case(spin, mcut)
+
+
+@pytest.mark.parametrize("spin", [0, 2])
+def test_ring_sht_polar_underflow_every_variant(spin, monkeypatch):
+    """The same check with each Legendre synthesis kernel variant forced (plan creation otherwise keeps
+    only the fastest one)."""
+    nv = _lib().load().nusht_legendre_variants()
+    assert nv >= 1
+    for v in range(nv):
+        monkeypatch.setenv("NUSHT_LEGENDRE_VARIANT", str(v))
+        _polar_underflow_case(spin, 40, variant=v)
+
+
+def _polar_underflow_case(spin, mcut, variant=None):
     """Rings 1e-4 .. 0.05 rad from both poles at lmax 320, alm zero for m < mcut: at such rings the
     Wigner-d values of large m start far below 2^-300 (scaled recursion, l-ranges that are skipped
     because a whole group of rings is still under range, partly masked groups).  Each ring's
@@ -199,6 +214,8 @@ def test_ring_sht_polar_underflow(spin, mcut):
     phi0 = rng.uniform(0, 2 * np.pi, nring)
     geom = Geom(theta, phi0, np.full(nring, nphi), np.arange(nring) * nphi, np.ones(nring))
     r = RingSHT(geom, L, spin)
+    if variant is not None:
+        assert r.info()["legendre_variant"] == variant
     nc = 1 if spin == 0 else 2
     a = almutil.random_alm(L, spin, nc, rng)
     mofa = np.concatenate([np.full(L + 1 - m, m) for m in range(L + 1)])  # m of each alm index
@@ -343,3 +360,42 @@ def test_batch_matches_single_calls(nmaps):
         _lib().call("nusht_nusht2d2", plan._h, _p(ab), _p(ref[0]), _p(ref[1]), _p(gam), _stream())
         _sync()
         assert torch.equal(out[b], ref) or _relmax(out[b].cpu().numpy(), ref.cpu().numpy()) < 1e-13
+
+
+@pytest.mark.parametrize("spin", [0, 2])
+def test_batch_every_variant(spin, monkeypatch):
+    """Every Legendre synthesis variant, batched (2 maps per launch) and single, at lmax 200 (two groups of
+    ring pairs, underflow near the poles), against variant 0's single-map result, within 1e-12 relative."""
+    from cunusht_amd.transformer import NuPlan
+    rng = np.random.default_rng(500 + spin)
+    L, npts, nmaps = 200, 3000, 3
+    nc = 1 if spin == 0 else 2
+    loc = np.stack([np.arccos(rng.uniform(-1, 1, npts)), rng.uniform(0, 2 * np.pi, npts)], 1)
+    th, ph = _t(loc[:, 0]), _t(loc[:, 1])
+    gam = _t(rng.uniform(-0.1, 0.1, npts)) if spin else None
+    a = np.stack([almutil.random_alm(L, spin, nc, rng) for _ in range(nmaps)])
+    at = _t(a)
+    nalm = a.shape[2]
+    nv = _lib().load().nusht_legendre_variants()
+    ref = None
+    for v in range(nv):
+        monkeypatch.setenv("NUSHT_LEGENDRE_VARIANT", str(v))
+        plan = NuPlan(L, spin, nc, 1e-10, npts)
+        assert plan.info()["legendre_variant"] == v
+        plan.set_pts(th, ph)
+        out = torch.zeros((nmaps, 2, npts), dtype=torch.float64, device="cuda")
+        _lib().call("nusht_nusht2d2_batch", plan._h, nmaps, _p(at), 2 * nc * nalm, _p(out[0, 0]),
+                    _p(out[0, 1]) if spin else None, 2 * npts, _p(gam) if spin else None, _stream())
+        single = torch.zeros((nmaps, 2, npts), dtype=torch.float64, device="cuda")
+        for b in range(nmaps):
+            ab = _t(a[b])
+            _lib().call("nusht_nusht2d2", plan._h, _p(ab), _p(single[b, 0]), _p(single[b, 1]) if spin else None,
+                        _p(gam) if spin else None, _stream())
+        _sync()
+        got_b, got_s = out.cpu().numpy(), single.cpu().numpy()
+        if ref is None:
+            ref = got_s
+        for b in range(nmaps):
+            assert _relmax(got_s[b], ref[b]) < 1e-12, (v, b)
+            assert _relmax(got_b[b], ref[b]) < 1e-12, (v, b)
+        del plan
