@@ -55,9 +55,10 @@ def stage_work(stage: str, info: dict, lmax: int, npix: int, maps_per_launch: in
     """(flops or bytes, unit) of one launch of a stage: the algorithmic work per unit times the units
     one launch processes.
 
-    Legendre synthesis: N_lm * ceil(n_rings / 2) (l, m, ring pair) units per map at 22 flops for a
-    spin-2 E+B map and 14 for the spin-1 gradient pair.  The kernels skip l-ranges in which a whole
-    group of 64 ring pairs is below 2^-300; `executed_steps` reports what they run.
+    Legendre synthesis: (l, m, ring pair) units per map at 22 flops for a spin-2 E+B map and 14 for
+    the spin-1 gradient pair.  The units counted are the ones a launch processes (the plan's
+    legendre_pair_steps): l-ranges in which a whole group of 64 ring pairs is below 2^-300 are
+    skipped.  The full grid, N_lm * ceil(n_rings / 2) units, is reported beside it.
     nuFFT interpolation: N_pts (16 + 16) + nrow nf b_grid bytes (FP64 coordinates, complex FP64
     values; the fine rows actually formed)."""
     nlm = (lmax + 1) * (lmax + 2) // 2
@@ -65,9 +66,9 @@ def stage_work(stage: str, info: dict, lmax: int, npix: int, maps_per_launch: in
     single = info["single_precision"]
     bc = 8 if single else 16
     if stage == "legendre_synth_cc_s2":
-        return nlm * math.ceil(nt / 2) * 22 * maps_per_launch, "flop"
+        return info["legendre_pair_steps"] * 22 * maps_per_launch, "flop"
     if stage == "legendre_synth_ring_s1" and rinfo is not None:
-        return nlm * math.ceil(rinfo["nring"] / 2) * 14, "flop"
+        return rinfo["legendre_pair_steps"] * 14, "flop"
     if stage == "nufft_interp":
         return npix * (16 + 16) + info["nrow_fine"] * info["nfine"] * bc, "byte"
     if stage == "theta_pass_t2":
@@ -267,7 +268,9 @@ def main():
             ach = work / avg_s / 1e12
             roof = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                     "frac": ach / PEAK_FP64_TFLOPS, "traffic": traffic, "kernel": name, "avg_ms": tot_ms / calls,
-                    "maps_per_launch": mpl, "algorithmic_flop": work}
+                    "maps_per_launch": mpl, "algorithmic_flop": work,
+                    "full_grid_flop": (L + 1) * (L + 2) // 2 * math.ceil(info["ntheta"] / 2) * 22 * mpl
+                    if name == "legendre_synth_cc_s2" else None}
             ex = executed_flops(name, info, mpl, rinfo)
             if ex is not None:
                 roof["executed_flop"] = ex

@@ -38,18 +38,21 @@ struct LegendreTables {
   DevBuf st_p;            // double2[(lmax+1) * sstride]  (mu_u, mu_v) at l = start - 1 (0 at l0)
   DevBuf st_k;            // double2[(lmax+1) * sstride]  scale exponents (k_u, k_v) at l = start
   DevBuf start;           // int[(lmax+1) * ngroups]
+  DevBuf mblk;            // int64[lmax+1]  first block of 4 l of each m in the matrix-core coefficient table
+  long nblk = 0;          // sum over m of ceil((lmax - l0(m) + 1) / 4)
   long nlm = 0;           // sum over m of (lmax - l0(m) + 1)
   long pair_steps = 0;    // (l, ring pair) recursion steps one synthesis runs (from `start`)
   std::string tag = "rings";  // stage-timer label ("cc" for the nuSHT grid, "ring" for the pointing grid)
   std::string n_prep, n_synth, n_adj;
-  int chunk = 2;          // maps per synthesis launch
+  int chunk = 2;          // maps per synthesis launch (VALU kernels)
+  bool mfma = true;       // batches of >= 3 maps on the matrix cores (NUSHT_LEGENDRE_MFMA=0 turns it off)
   int variant = -1;       // synthesis kernel variant (-1: default); set by legendre_autotune
   float tuned_ms = 0.f;
 
   void build(int lmax, int spin, const std::vector<double>& ring_theta, hipStream_t st);
   size_t bytes() const {
     return slot_theta.bytes() + slot_north.bytes() + slot_south.bytes() + mofs.bytes() + rec.bytes() +
-           cnorm.bytes() + st_mu.bytes() + st_p.bytes() + st_k.bytes() + start.bytes();
+           cnorm.bytes() + st_mu.bytes() + st_p.bytes() + st_k.bytes() + start.bytes() + mblk.bytes();
   }
 };
 
@@ -59,6 +62,8 @@ struct LegendreTables {
 // (negative m -> lmax - |m|); element (row, ring) at G + row * g_stride_m + ring * g_stride_r (+ map * g_stride_map).
 // fl (optional, device double[lmax+1]) multiplies E by fl[l] before synthesis.
 size_t legendre_coef_bytes(const LegendreTables& t, int nmaps);
+// Maps per launch of the matrix-core synthesis path (used for batches of at least 3 maps).
+constexpr int kLegendreMfmaMaps = 4;
 void legendre_synth(const LegendreTables& t, int nmaps, int ncomp, const double2* alm, long alm_stride_map,
                     const double* fl, double2* coef_ws, double2* G, long g_stride_m, long g_stride_r,
                     long g_stride_map, hipStream_t st);

@@ -402,6 +402,224 @@ __global__ __launch_bounds__(64 * WPB) void k_synth(SynthArgs a) {
   }
 }
 
+// ------------------------------------------------------------------ synthesis on the matrix cores
+// For 4 maps per launch the contraction over l is a GEMM per m: per chunk of 4 l,
+// (64 ring pairs x 4 l) x (4 l x 16 columns), done with v_mfma_f64_16x16x4_f64 while the VALU
+// runs the recursion.  Columns, 4 per map b, with s_l = (-1)^{l+m}:
+//   spin s, u chain: Re W_b, Im W_b, Re s_l Wt_b, Im s_l Wt_b    (north +m, south -m)
+//           v chain: Re Wt_b, Im Wt_b, Re s_l W_b, Im s_l W_b    (north -m, south +m)
+//   spin 0, u chain: Re a_b, Im a_b, Re s_l a_b, Im s_l a_b      (north, south)
+// The table holds, per m and per block q of 4 l (l = l0 + 4q + k), 64 doubles per chain in the
+// B-operand order of the instruction: lane j + 16k holds column j of l0 + 4q + k.  Entries past
+// lmax are zero.
+
+// Operand layouts of v_mfma_f64_16x16x4_f64 on gfx950 (64 lanes):
+//   A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in register i / 4 of lane j + 16 (i % 4).
+
+__global__ void k_prep4(int lmax, int s, int nmaps, int ncomp, const double2* __restrict__ alm, long stride_map,
+                        const double* __restrict__ fl, const long* __restrict__ mofs, const long* __restrict__ mblk,
+                        const double* __restrict__ cnorm, double* __restrict__ coef) {
+  // one thread per table element (coalesced stores): e = ((q * nc + chain) * 4 + k) * 16 + j
+  const int m = blockIdx.y;
+  const int l0 = l0_of(m, s);
+  if (l0 > lmax) return;
+  const int nl = lmax - l0 + 1;
+  const int nc = s ? 2 : 1;
+  const long nel = (long)((nl + 3) >> 2) * nc * 64;
+  const long nalm = (long)(lmax + 1) * (lmax + 2) / 2;
+  const long mbase = (long)m * (2 * lmax + 1 - m) / 2;
+  const double sg = (s & 1) ? 1.0 : -1.0;  // -(-1)^s
+  double* out = coef + mblk[m] * nc * 64;
+  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nel; e += (long)gridDim.x * blockDim.x) {
+    const int j = (int)(e & 15), k = (int)((e >> 4) & 3);
+    const long qc = e >> 6;
+    const int chain = (int)(qc % nc);
+    const long q = qc / nc;
+    const int jj = (int)(4 * q + k);
+    const int l = l0 + jj;
+    const int b = j >> 2, c = j & 3;
+    double v = 0.0;
+    if (l <= lmax && b < nmaps) {
+      double cl = cnorm[mofs[m] + jj];
+      if (fl) cl *= fl[l];
+      const double2* ab = alm + b * stride_map;
+      const double2 ev = ab[mbase + l];
+      const double2 bv = (s != 0 && ncomp > 1) ? ab[nalm + mbase + l] : double2{0.0, 0.0};
+      const double sl = ((l + m) & 1) ? -1.0 : 1.0;
+      double2 z;
+      if (s == 0) {
+        z = double2{cl * ev.x, cl * ev.y};
+      } else {
+        const double2 w{sg * cl * (ev.x - bv.y), sg * cl * (ev.y + bv.x)};
+        const double2 wt{-cl * (ev.x + bv.y), -cl * (-ev.y + bv.x)};
+        // u chain: W, s_l Wt;  v chain: Wt, s_l W
+        z = (chain == 0) == (c < 2) ? w : wt;
+      }
+      const double f = c < 2 ? 1.0 : sl;
+      v = f * ((c & 1) ? z.y : z.x);
+    }
+    out[e] = v;
+  }
+}
+
+typedef double v4d_t __attribute__((ext_vector_type(4)));
+
+// 4 x 4 transpose of (register k, row g of 16 lanes): afterwards register g holds in lane i + 16k
+// what register k held in lane 16g + i (the A-operand tile of ring pairs 16g .. 16g + 15).
+__device__ __forceinline__ void transpose_rows(double (&u)[4]) {
+  unsigned lo[4], hi[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    lo[k] = (unsigned)__double2loint(u[k]);
+    hi[k] = (unsigned)__double2hiint(u[k]);
+  }
+  auto sw32 = [](unsigned& x, unsigned& y) {
+    auto r = __builtin_amdgcn_permlane32_swap(x, y, false, false);  // x upper half <-> y lower half
+    x = r[0]; y = r[1];
+  };
+  auto sw16 = [](unsigned& x, unsigned& y) {
+    auto r = __builtin_amdgcn_permlane16_swap(x, y, false, false);  // x odd rows <-> y even rows
+    x = r[0]; y = r[1];
+  };
+  sw32(lo[0], lo[2]); sw32(lo[1], lo[3]); sw16(lo[0], lo[1]); sw16(lo[2], lo[3]);
+  sw32(hi[0], hi[2]); sw32(hi[1], hi[3]); sw16(hi[0], hi[1]); sw16(hi[2], hi[3]);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) u[k] = __hiloint2double((int)hi[k], (int)lo[k]);
+}
+
+template <bool SP>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_synth_mfma(SynthArgs a, const double* __restrict__ coef4,
+                                                   const long* __restrict__ mblk) {
+  constexpr int NC = SP ? 2 : 1;
+  // same block -> (m, slot group) mapping as k_synth with one wave per block
+  const int blk = blockIdx.x;
+  const int j = blk / kNumXcd;
+  const int m = (blk - j * kNumXcd) + kNumXcd * (j / a.nsb);
+  const int sb = j % a.nsb;
+  const int L = a.lmax;
+  if (m > L) return;
+  const int l0 = l0_of(m, a.s);
+  if (l0 > L || sb * 64 >= a.nslot) return;
+  const int lane = threadIdx.x & 63;
+  const int lstart = a.start[(long)m * ((a.nslot + 63) >> 6) + sb];
+
+  const int slot = sb * 64 + lane;
+  double x = 0, mu_u = 0, mu_v = 0, pu = 0, pv = 0, ku = 0, kv = 0;
+  if (slot < a.nslot) {
+    const long q = (long)m * a.sstride + slot;
+    x = cos(a.slot_theta[slot]);
+    const double2 sm = a.st_mu[q], sp = a.st_p[q], sk = a.st_k[q];
+    mu_u = sm.x; pu = sp.x; ku = sk.x;
+    if (SP) { mu_v = sm.y; pv = sp.y; kv = sk.y; }
+  }
+  const double2* __restrict__ rec = a.rec + a.mofs[m];
+  const double* __restrict__ cb = coef4 + mblk[m] * NC * 64;
+
+  v4d_t Du[4], Dv[4];
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    Du[g] = v4d_t{0.0, 0.0, 0.0, 0.0};
+    Dv[g] = v4d_t{0.0, 0.0, 0.0, 0.0};
+  }
+  struct Buf {
+    double2 rc[4];
+    double bu, bv;
+  };
+  auto load = [&](int lc, Buf& b) {
+    const int jl = lc - l0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) b.rc[i] = rec[jl + i];
+    const double* blkp = cb + (long)(jl >> 2) * NC * 64;
+    b.bu = blkp[lane];
+    if (SP) b.bv = blkp[64 + lane];
+  };
+  auto chunk = [&](int lc, const Buf& bf, auto slow_tag, auto full_tag) {
+    constexpr bool SLOW = decltype(slow_tag)::value;
+    constexpr bool FULL = decltype(full_tag)::value;
+    double U[4], V[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      if (!FULL && lc + i > L) { U[i] = 0.0; V[i] = 0.0; continue; }  // zero weight past lmax
+      double wu = mu_u, wv = mu_v;
+      if (SLOW) { wu = ku == 0.0 ? wu : 0.0; wv = kv == 0.0 ? wv : 0.0; }
+      U[i] = wu; V[i] = wv;
+      const double t = bf.rc[i].x * x;
+      const double nu = fma(t + bf.rc[i].y, mu_u, -pu);
+      pu = mu_u; mu_u = nu;
+      if (SP) {
+        const double nv = fma(t - bf.rc[i].y, mu_v, -pv);
+        pv = mu_v; mu_v = nv;
+      }
+      if (SLOW) {
+        if (ku < 0.0 && fabs(mu_u) > kThresh) { mu_u *= kSmall; pu *= kSmall; ku += 1.0; }
+        if (SP && kv < 0.0 && fabs(mu_v) > kThresh) { mu_v *= kSmall; pv *= kSmall; kv += 1.0; }
+      }
+    }
+    transpose_rows(U);
+#pragma unroll
+    for (int g = 0; g < 4; ++g) Du[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(U[g], bf.bu, Du[g], 0, 0, 0);
+    if (SP) {
+      transpose_rows(V);
+#pragma unroll
+      for (int g = 0; g < 4; ++g) Dv[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(V[g], bf.bv, Dv[g], 0, 0, 0);
+    }
+  };
+  auto any_under = [&]() { return __any((ku < 0.0) || (kv < 0.0)); };
+  using T_ = std::true_type;
+  using F_ = std::false_type;
+  int lc = lstart;
+  Buf cur, nxt;
+  load(lc, cur);
+  // slow phase (some lane below 2^-300), fast phase, partial last chunk; the next chunk's
+  // coefficients are loaded before the current chunk is computed
+  while (lc <= L && any_under()) {
+    load(lc + 4, nxt);
+    if (lc + 3 <= L) chunk(lc, cur, T_{}, T_{});
+    else chunk(lc, cur, T_{}, F_{});
+    cur = nxt;
+    lc += 4;
+  }
+  while (lc + 3 <= L) {
+    load(lc + 4, nxt);
+    chunk(lc, cur, F_{}, T_{});
+    cur = nxt;
+    lc += 4;
+  }
+  if (lc <= L) chunk(lc, cur, F_{}, F_{});
+
+  // epilogue: lane = jc + 16 qq holds column jc for ring pairs 16 g + 4 r + qq (register r)
+  const int jc = lane & 15, qq = lane >> 4;
+  const int b = jc >> 2, c = jc & 3;
+  if (b >= a.nmaps) return;
+  double* Gd = reinterpret_cast<double*>(a.G + b * a.smap);
+  const int part = c & 1;
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int sl = sb * 64 + 16 * g + 4 * r + qq;
+      if (sl >= a.nslot) continue;
+      const int rn = a.slot_north[sl];
+      const int rs = a.slot_south[sl];
+      const double du = Du[g][r];
+      if (!SP) {
+        if (c < 2) Gd[((long)m * a.sm + (long)rn * a.sr) * 2 + part] = du;
+        else if (rs >= 0) Gd[((long)m * a.sm + (long)rs * a.sr) * 2 + part] = du;
+      } else {
+        const double dv = Dv[g][r];
+        const long rowp = (long)(L + m) * a.sm, rowm = (long)(L - m) * a.sm;
+        if (c < 2) {
+          Gd[(rowp + (long)rn * a.sr) * 2 + part] = du;                        // north +m
+          if (m > 0) Gd[(rowm + (long)rn * a.sr) * 2 + part] = dv;            // north -m
+        } else if (rs >= 0) {
+          Gd[(rowp + (long)rs * a.sr) * 2 + part] = dv;                        // south +m
+          if (m > 0) Gd[(rowm + (long)rs * a.sr) * 2 + part] = du;            // south -m
+        }
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------ adjoint kernel
 template <bool SP> struct AdjCfg {
   static constexpr int R = SP ? 3 : 4;          // ring pairs per lane per pass
@@ -641,6 +859,7 @@ __global__ __launch_bounds__(AdjCfg<SP>::MaxThreads) void k_adjoint(AdjArgs a) {
 // ------------------------------------------------------------------ host
 void LegendreTables::build(int L, int s, const std::vector<double>& theta, hipStream_t st) {
   NUSHT_REQUIRE(L >= 0 && s >= 0 && s <= 3, kErrArg, "legendre: bad lmax/spin");
+  if (const char* e = getenv("NUSHT_LEGENDRE_MFMA"); e && e[0] == '0') mfma = false;
   lmax = L;
   spin = s;
   nring = (int)theta.size();
@@ -687,6 +906,13 @@ void LegendreTables::build(int L, int s, const std::vector<double>& theta, hipSt
   }
   nlm = mo[L + 1];
   upload(mofs, mo);
+  std::vector<long> mb(L + 1, 0);
+  nblk = 0;
+  for (int m = 0; m <= L; ++m) {
+    mb[m] = nblk;
+    nblk += ceil_div((long)(mo[m + 1] - mo[m]), 4L);
+  }
+  upload(mblk, mb);
   // padded so that a chunked, prefetching l-loop may read up to 2 kSynCH entries past the last one
   rec.alloc(sizeof(double2) * (size_t)(std::max<long>(nlm, 1) + 2 * kSynCH));
   cnorm.alloc(sizeof(double) * (size_t)(std::max<long>(nlm, 1) + 2 * kSynCH));
@@ -729,7 +955,12 @@ static int padded_maps(int nmaps) { return nmaps <= 1 ? 1 : nmaps <= 2 ? 2 : nma
 
 size_t legendre_coef_bytes(const LegendreTables& t, int nmaps) {
   // padded by 2 kSynCH entries (chunked, prefetching reads past the last l of the last m)
-  return sizeof(double2) * (size_t)(std::max<long>(t.nlm, 1) + 2 * kSynCH) * padded_maps(nmaps) * (t.spin ? 2 : 1);
+  const size_t vec = sizeof(double2) * (size_t)(std::max<long>(t.nlm, 1) + 2 * kSynCH) * padded_maps(nmaps) *
+                     (t.spin ? 2 : 1);
+  if (nmaps < 3) return vec;
+  // matrix-core layout: 64 doubles per chain per block of 4 l, 2 blocks of padding (prefetch)
+  const size_t mf = sizeof(double) * 64 * (t.spin ? 2 : 1) * (size_t)(t.nblk + 2);
+  return std::max(vec, mf);
 }
 
 // Synthesis variants (l-chunk CH, waves per block WPB); the autotuner picks one per table.
@@ -763,15 +994,65 @@ static void legendre_synth_chunk(const LegendreTables& t, int nmaps, int ncomp, 
                                  long alm_stride_map, const double* fl, double2* coef_ws, double2* G, long sm, long sr,
                                  long smap, hipStream_t st);
 
+static void legendre_synth_mfma(const LegendreTables& t, int nmaps, int ncomp, const double2* alm,
+                                long alm_stride_map, const double* fl, double2* coef_ws, double2* G, long sm, long sr,
+                                long smap, hipStream_t st);
+
 void legendre_synth(const LegendreTables& t, int nmaps, int ncomp, const double2* alm, long alm_stride_map,
                     const double* fl, double2* coef_ws, double2* G, long sm, long sr, long smap, hipStream_t st) {
   NUSHT_REQUIRE(nmaps >= 1 && nmaps <= 8, kErrArg, "legendre_synth: nmaps must be in [1, 8]");
   const int ch = std::max(1, std::min(t.chunk, 2));
-  for (int b = 0; b < nmaps; b += ch) {
-    const int nb = std::min(ch, nmaps - b);
-    legendre_synth_chunk(t, nb, ncomp, alm + b * alm_stride_map, alm_stride_map, fl, coef_ws, G + b * smap, sm, sr,
-                         smap, st);
+  int b = 0;
+  while (b < nmaps) {
+    const int left = nmaps - b;
+    if (t.mfma && left >= 3) {
+      const int nb = std::min(kLegendreMfmaMaps, left);
+      legendre_synth_mfma(t, nb, ncomp, alm + b * alm_stride_map, alm_stride_map, fl, coef_ws, G + b * smap, sm, sr,
+                          smap, st);
+      b += nb;
+    } else {
+      const int nb = std::min(ch, left);
+      legendre_synth_chunk(t, nb, ncomp, alm + b * alm_stride_map, alm_stride_map, fl, coef_ws, G + b * smap, sm,
+                           sr, smap, st);
+      b += nb;
+    }
   }
+}
+
+static void legendre_synth_mfma(const LegendreTables& t, int nmaps, int ncomp, const double2* alm,
+                                long alm_stride_map, const double* fl, double2* coef_ws, double2* G, long sm, long sr,
+                                long smap, hipStream_t st) {
+  const int L = t.lmax;
+  double* coef4 = reinterpret_cast<double*>(coef_ws);
+  {
+    StageScope sc(t.n_prep.c_str(), st);
+    const int nc = t.spin ? 2 : 1;
+    dim3 grid(ceil_div((L + 4) / 4 * nc * 64, 256), L + 1);
+    hipLaunchKernelGGL(k_prep4, grid, dim3(256), 0, st, L, t.spin, nmaps, ncomp, alm, alm_stride_map, fl,
+                       t.mofs.as<long>(), t.mblk.as<long>(), t.cnorm.as<double>(), coef4);
+    NUSHT_LAUNCH_CHECK();
+  }
+  SynthArgs a;
+  a.lmax = L; a.s = t.spin; a.nslot = t.nslot; a.sstride = t.sstride; a.nmaps = nmaps;
+  a.nsb = ceil_div(a.nslot, 64) | 1;
+  a.slot_theta = t.slot_theta.as<double>();
+  a.slot_north = t.slot_north.as<int>();
+  a.slot_south = t.slot_south.as<int>();
+  a.mofs = t.mofs.as<long>();
+  a.rec = t.rec.as<double2>();
+  a.st_mu = t.st_mu.as<double2>();
+  a.st_p = t.st_p.as<double2>();
+  a.st_k = t.st_k.as<double2>();
+  a.start = t.start.as<int>();
+  a.coef = coef_ws;
+  a.G = G; a.sm = sm; a.sr = sr; a.smap = smap;
+  const int nblocks = a.nsb * ceil_div(L + 1, kNumXcd) * kNumXcd;
+  StageScope sc(t.n_synth.c_str(), st);
+  if (t.spin > 0)
+    hipLaunchKernelGGL(k_synth_mfma<true>, dim3(nblocks), dim3(64), 0, st, a, coef4, t.mblk.as<long>());
+  else
+    hipLaunchKernelGGL(k_synth_mfma<false>, dim3(nblocks), dim3(64), 0, st, a, coef4, t.mblk.as<long>());
+  NUSHT_LAUNCH_CHECK();
 }
 
 static void legendre_synth_chunk(const LegendreTables& t, int nmaps, int ncomp, const double2* alm,

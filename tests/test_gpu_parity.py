@@ -399,3 +399,34 @@ def test_batch_every_variant(spin, monkeypatch):
             assert _relmax(got_s[b], ref[b]) < 1e-12, (v, b)
             assert _relmax(got_b[b], ref[b]) < 1e-12, (v, b)
         del plan
+
+
+@pytest.mark.parametrize("spin", [0, 2])
+@pytest.mark.parametrize("nmaps", [4, 5, 8])
+def test_batch_matrix_core_path(spin, nmaps, monkeypatch):
+    """Batches of >= 3 maps run the Legendre contraction on the matrix cores (4 maps per launch; the
+    remainder on the vector kernels).  Compared with the vector-only path (NUSHT_LEGENDRE_MFMA=0) at
+    lmax 200 (two groups of ring pairs, underflow near the poles), within 1e-12 relative."""
+    from cunusht_amd.transformer import NuPlan
+    rng = np.random.default_rng(700 + 10 * spin + nmaps)
+    L, npts = 200, 3000
+    nc = 1 if spin == 0 else 2
+    loc = np.stack([np.arccos(rng.uniform(-1, 1, npts)), rng.uniform(0, 2 * np.pi, npts)], 1)
+    th, ph = _t(loc[:, 0]), _t(loc[:, 1])
+    gam = _t(rng.uniform(-0.1, 0.1, npts)) if spin else None
+    a = np.stack([almutil.random_alm(L, spin, nc, rng) for _ in range(nmaps)])
+    at = _t(a)
+    nalm = a.shape[2]
+    res = {}
+    for mf in ("1", "0"):
+        monkeypatch.setenv("NUSHT_LEGENDRE_MFMA", mf)
+        plan = NuPlan(L, spin, nc, 1e-10, npts)
+        plan.set_pts(th, ph)
+        out = torch.zeros((nmaps, 2, npts), dtype=torch.float64, device="cuda")
+        _lib().call("nusht_nusht2d2_batch", plan._h, nmaps, _p(at), 2 * nc * nalm, _p(out[0, 0]),
+                    _p(out[0, 1]) if spin else None, 2 * npts, _p(gam) if spin else None, _stream())
+        _sync()
+        res[mf] = out.cpu().numpy()
+        del plan
+    for b in range(nmaps):
+        assert _relmax(res["1"][b], res["0"][b]) < 1e-12, b
