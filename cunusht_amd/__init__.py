@@ -6,9 +6,11 @@ nusht2d2, nusht2d1, gclm2lenmap, lenmap2gclm and dlm2pointing (cunusht/cunusht.p
 from __future__ import annotations
 
 from .geometry import Geom, get_geom, good_size
+from .helpers import dlm2A, get_spin_lower, get_spin_raise
 from .utils import Alm, almxfl, synalm
 
-__all__ = ["get_transformer", "get_geom", "Geom", "Alm", "almxfl", "synalm", "good_size"]
+__all__ = ["get_transformer", "get_geom", "Geom", "Alm", "almxfl", "synalm", "good_size", "dlm2A",
+           "get_spin_raise", "get_spin_lower"]
 
 
 def get_transformer(backend: str, solver: str = "default", mode: str = "nuFFT"):

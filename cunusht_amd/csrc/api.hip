@@ -91,6 +91,22 @@ int nusht_pointing_f64(const double* theta, const double* phi0, const int64_t* n
   });
 }
 
+int nusht_magnification_f64(const double* kappa, const double* g1, const double* g2, const double* d1,
+                            const double* d2, const double* omega, int64_t npix, int exact, double* A,
+                            nusht_stream_t s) {
+  return guard([&] {
+    NUSHT_REQUIRE(kappa && g1 && g2 && d1 && d2 && A, kErrArg, "null array");
+    magnification(kappa, g1, g2, d1, d2, omega, npix, exact != 0, A, S(s));
+  });
+}
+
+int nusht_almxfl_c128(const double* alm, int lmax, int ncomp, const double* fl, double* out, nusht_stream_t s) {
+  return guard([&] {
+    NUSHT_REQUIRE(alm && fl && out, kErrArg, "null array");
+    almxfl(reinterpret_cast<const double2*>(alm), lmax, ncomp, fl, reinterpret_cast<double2*>(out), S(s));
+  });
+}
+
 // ------------------------------------------------------------------ ring plan
 int nusht_ringplan_create(const nusht_ring_desc* d, nusht_stream_t s, nusht_ringplan_t* out) {
   return guard([&] {

@@ -41,6 +41,12 @@ void doubling(const T* map, int ntheta, int nphi, int spin, T* dmap, hipStream_t
 template <typename T>
 void adjoint_doubling(const T* dmap, int ntheta, int nphi, int spin, T* map, hipStream_t st);
 
+// Lensing magnification determinant from convergence, shear, deflection and curl fields (w may be null).
+void magnification(const double* k, const double* g1, const double* g2, const double* d1, const double* d2,
+                   const double* w, long n, bool exact, double* out, hipStream_t st);
+// out = fl[l] * alm (ncomp blocks of nalm complex; out may alias alm).
+void almxfl(const double2* alm, int lmax, int ncomp, const double* fl, double2* out, hipStream_t st);
+
 void pointing(const double* theta, const double* phi0, const long* nph, const long* ofs, int nring, long max_nph,
               const double* dt, const double* dp, double* pt, double* pp, double* gamma, hipStream_t st);
 

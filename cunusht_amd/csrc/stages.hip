@@ -12,6 +12,8 @@
 // (CPU_nufft_transformer.py:158-160).
 #include "stages.h"
 
+#include <algorithm>
+
 #include <mutex>
 
 namespace nusht {
@@ -111,6 +113,67 @@ template void adjoint_doubling<float>(const float*, int, int, int, float*, hipSt
 template void adjoint_doubling<double>(const double*, int, int, int, double*, hipStream_t);
 template void adjoint_doubling<float2>(const float2*, int, int, int, float2*, hipStream_t);
 template void adjoint_doubling<double2>(const double2*, int, int, int, double2*, hipStream_t);
+
+// ------------------------------------------------------------------ lensing magnification
+// Determinant of the magnification matrix of the deflection (cunusht/helper_GPU.py:113-161):
+//   A = f0 ((1-k)^2 - g1^2 - g2^2 + w^2) + f1 (1-k) + sg (f1/|d|^2) ((d1^2 - d2^2) g1 + 2 d1 d2 g2),
+//   f0 = sin|d|/|d|, f1 = cos|d| - f0, series for |d| < 1e-4 (f0 = 1 - d^2/6, f1/d^2 = -1/3 + d^2/30).
+// sg = +1 gives the solid-angle ratio dOmega'/dOmega of the map n -> cos|d| n + sin|d| d_hat
+// exactly; sg = -1 is the reference's sign, correct through second order in |d|.
+__global__ void k_magnification(const double* __restrict__ k, const double* __restrict__ g1,
+                                const double* __restrict__ g2, const double* __restrict__ d1,
+                                const double* __restrict__ d2, const double* __restrict__ w, long n, double sg,
+                                double* __restrict__ out) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const double a = d1[i], b = d2[i], kk = k[i], s1 = g1[i], s2 = g2[i];
+    const double ww = w ? w[i] : 0.0;
+    const double dd = a * a + b * b;
+    const double d = sqrt(dd);
+    double f0, f1, f1od2;
+    if (d < 1e-4) {
+      f0 = 1.0 - dd / 6.0;
+      f1od2 = -1.0 / 3.0 + dd / 30.0;
+      f1 = f1od2 * dd;
+    } else {
+      f0 = sin(d) / d;
+      f1 = cos(d) - f0;
+      f1od2 = f1 / dd;
+    }
+    const double omk = 1.0 - kk;
+    out[i] = f0 * (omk * omk - s1 * s1 - s2 * s2 + ww * ww) + f1 * omk +
+             sg * f1od2 * ((a * a - b * b) * s1 + 2.0 * a * b * s2);
+  }
+}
+
+void magnification(const double* k, const double* g1, const double* g2, const double* d1, const double* d2,
+                   const double* w, long n, bool exact, double* out, hipStream_t st) {
+  NUSHT_REQUIRE(n >= 0, kErrArg, "magnification: negative size");
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_magnification, dim3((unsigned)std::min<long>(ceil_div(n, 256), 65536)), dim3(256), 0, st, k,
+                     g1, g2, d1, d2, w, n, exact ? 1.0 : -1.0, out);
+  NUSHT_LAUNCH_CHECK();
+}
+
+// out_lm = fl[l] alm_lm over the healpy m-major layout (out may alias alm)
+__global__ void k_almxfl(const double2* __restrict__ alm, int lmax, int ncomp, const double* __restrict__ fl,
+                         double2* __restrict__ out) {
+  const int m = blockIdx.y;
+  const long nalm = (long)(lmax + 1) * (lmax + 2) / 2;
+  const long base = (long)m * (2 * lmax + 1 - m) / 2;
+  for (int l = m + blockIdx.x * blockDim.x + threadIdx.x; l <= lmax; l += gridDim.x * blockDim.x) {
+    const double f = fl[l];
+    for (int c = 0; c < ncomp; ++c) {
+      const double2 v = alm[c * nalm + base + l];
+      out[c * nalm + base + l] = double2{v.x * f, v.y * f};
+    }
+  }
+}
+
+void almxfl(const double2* alm, int lmax, int ncomp, const double* fl, double2* out, hipStream_t st) {
+  NUSHT_REQUIRE(lmax >= 0 && ncomp >= 1, kErrArg, "almxfl: bad lmax or ncomp");
+  hipLaunchKernelGGL(k_almxfl, dim3(ceil_div(lmax + 1, 256), lmax + 1), dim3(256), 0, st, alm, lmax, ncomp, fl, out);
+  NUSHT_LAUNCH_CHECK();
+}
 
 void pointing(const double* theta, const double* phi0, const long* nph, const long* ofs, int nring, long max_nph,
               const double* dt, const double* dp, double* pt, double* pp, double* gamma, hipStream_t st) {

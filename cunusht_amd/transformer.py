@@ -257,6 +257,15 @@ class GPU_nusht_transformer:
                   _ptr(out[2] if gamma else None), _stream())
         return out
 
+    def dlm2A(self, dlm, dclm=None, exact: bool = True):
+        """Determinant of the magnification matrix of the deflection with gradient dlm (and curl dclm) on
+        the pointing grid (cunusht/helper_GPU.py:113-161).  Note: dlm, not dlm_scaled.  See helpers.dlm2A."""
+        from .helpers import magnification_from_alm
+        dlm = _dev(dlm, torch.complex128, "dlm").reshape(-1)
+        assert dlm.numel() == Alm.getsize(self.lmax, self.lmax), "dlm must have the plan's lmax"
+        dcl = None if dclm is None else _dev(dclm, torch.complex128, "dclm").reshape(-1)
+        return magnification_from_alm(self, dlm, dcl, exact)
+
     def _pointing(self, dlm_scaled, ptg, want_gamma):
         if ptg is not None:
             ptg = _dev(ptg, torch.float64, "ptg")

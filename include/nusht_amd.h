@@ -65,6 +65,19 @@ int nusht_pointing_f64(const double* theta, const double* phi0, const int64_t* n
                        int nring, int64_t max_nph, const double* d_theta, const double* d_phi,
                        double* out_theta, double* out_phi, double* out_gamma, nusht_stream_t s);
 
+/* Determinant of the lensing magnification matrix per pixel.  Replaces the pointwise part of
+ * cunusht/helper_GPU.py:113-161 (dlm2A); kappa, g1, g2, d1, d2, omega (omega may be NULL) are the
+ * syntheses on the pointing grid of 0.5 sqrt(l(l+1)) dlm (spin 0), -0.5 sqrt((l-1)(l+2)) [dlm, dclm]
+ * (spin 2), [dlm, dclm] (spin 1) and 0.5 sqrt(l(l+1)) dclm (spin 0).  exact != 0: the solid-angle
+ * ratio of the deflection map; exact = 0: the reference's formula (shear sign of the |d|-dependent term
+ * as at helper_GPU.py:126; equal through second order in |d|).  |d| -> 0 is handled by series. */
+int nusht_magnification_f64(const double* kappa, const double* g1, const double* g2, const double* d1,
+                            const double* d2, const double* omega, int64_t npix, int exact, double* A,
+                            nusht_stream_t s);
+/* out = fl[l] alm (ncomp blocks of nalm complex in the layout above; out may alias alm).  Replaces the
+ * almxfl calls of helper_GPU.py:126-130 on the device. */
+int nusht_almxfl_c128(const double* alm, int lmax, int ncomp, const double* fl, double* out, nusht_stream_t s);
+
 /* ---------------------------------------------------------------- ring plan (pointing grid SHT)
  * Replaces the GL-grid SHTns transformer of cunusht/sht/GPU_sht_transformer.py:28-34 and its
  * synthesis_cupy (:56-62), adjoint_synthesis_cupy (:65-73, exact transpose, no weights) and

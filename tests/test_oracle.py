@@ -226,3 +226,48 @@ def test_pipeline_vs_direct_and_adjoint(spin):
     lhs = np.sum(x * out)
     rhs = sum(almutil.myalmdot(g[c], ad[c], L) for c in range(nc))
     assert abs(lhs - rhs) < 1e-9 * abs(lhs)  # test/test_adjointOperator.py:268 rtol
+
+
+# ------------------------------------------------------------------ magnification (dlm2A)
+def _lens_case(amp, curl, seed=3, L=12):
+    from oracle import almutil
+    rng = np.random.default_rng(seed)
+    ll = np.arange(L + 1)
+    cl = np.where(ll >= 1, 0.02 / (ll + 1.0) ** 2, 0.0)
+    dlm = almutil.synalm(cl, L, rng) * amp
+    dclm = almutil.synalm(cl, L, rng) * amp if curl else None
+    th = np.arccos(rng.uniform(-0.9, 0.9, 30))
+    ph = rng.uniform(0, 2 * np.pi, 30)
+    return dlm, dclm, L, th, ph
+
+
+@pytest.mark.parametrize("curl", [False, True])
+@pytest.mark.parametrize("amp", [0.01, 0.3, 1.0])
+def test_magnification_is_solid_angle_jacobian(amp, curl):
+    """exact=True equals dOmega'/dOmega of the deflection map (finite differences of the deflected
+    3-vector; |d| up to 0.26 rad) within the differencing error."""
+    from oracle import lensing
+    dlm, dclm, L, th, ph = _lens_case(amp, curl)
+    A = lensing.magnification(dlm, L, th, ph, dclm, exact=True)
+    R = lensing.solid_angle_ratio(dlm, L, th, ph, dclm, h=1e-3)
+    assert np.max(np.abs(A - R)) < 2e-9
+
+
+def test_magnification_reference_sign_is_second_order():
+    """The reference's shear sign (exact=False) agrees with the Jacobian through O(|d|^2): its error
+    grows as the cube of the deflection amplitude."""
+    from oracle import lensing
+    e = []
+    for amp in (0.01, 0.03):
+        dlm, dclm, L, th, ph = _lens_case(amp, True)
+        A = lensing.magnification(dlm, L, th, ph, dclm, exact=False)
+        R = lensing.solid_angle_ratio(dlm, L, th, ph, dclm, h=1e-3)
+        e.append(np.max(np.abs(A - R)))
+    assert 1e-9 < e[0] < 1e-7
+    assert 24 < e[1] / e[0] < 30
+
+
+def test_magnification_zero_deflection():
+    from oracle import lensing
+    z = np.zeros(5)
+    assert np.array_equal(lensing.combine(z, z, z, z, z, z), np.ones(5))
