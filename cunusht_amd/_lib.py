@@ -30,7 +30,7 @@ class NuShtError(RuntimeError):
 
 class RingDesc(ctypes.Structure):
     _fields_ = [("lmax", _i), ("spin", _i), ("nring", _i), ("theta", _vp), ("phi0", _vp), ("nph", _vp),
-                ("ofs", _vp)]
+                ("ofs", _vp), ("weight", _vp)]
 
 
 class PlanDesc(ctypes.Structure):
@@ -69,7 +69,11 @@ SIGNATURES = {
     "nusht_legendre_variants": [],
     "nusht_ringplan_synthesis": [_vp, _vp, _i, _vp, _vp, _vp],
     "nusht_ringplan_adjoint": [_vp, _vp, _vp, _vp, _i, _vp],
+    "nusht_ringplan_analysis": [_vp, _vp, _vp, _vp, _i, _vp],
     "nusht_ringplan_dlm2pointing": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "nusht_pointplan_create": [_i, _i, _i64, _vp, _vp, ctypes.POINTER(_vp)],
+    "nusht_pointplan_destroy": [_vp],
+    "nusht_pointplan_synthesis": [_vp, _vp, _i, _vp, _vp, _vp, _vp, _vp],
     "nusht_plan_create": [ctypes.POINTER(PlanDesc), _vp, ctypes.POINTER(_vp)],
     "nusht_plan_destroy": [_vp],
     "nusht_plan_get_info": [_vp, ctypes.POINTER(PlanInfo)],

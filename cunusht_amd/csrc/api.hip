@@ -7,6 +7,7 @@
 
 #include "legendre.h"
 #include "nufft.h"
+#include "points.h"
 #include "ring.h"
 #include "stages.h"
 #include "timing.h"
@@ -26,6 +27,10 @@ struct nusht_plan_s {
 
 struct nusht_ringplan_s {
   std::unique_ptr<RingPlan> rp;
+};
+
+struct nusht_pointplan_s {
+  std::unique_ptr<PointPlan> pp;
 };
 
 namespace {
@@ -107,6 +112,33 @@ int nusht_almxfl_c128(const double* alm, int lmax, int ncomp, const double* fl, 
   });
 }
 
+// ------------------------------------------------------------------ point plan
+int nusht_pointplan_create(int lmax, int spin, int64_t npts, const double* theta, nusht_stream_t s,
+                           nusht_pointplan_t* out) {
+  return guard([&] {
+    NUSHT_REQUIRE(theta && out, kErrArg, "null argument");
+    NUSHT_REQUIRE(lmax >= 0 && spin >= 0 && spin <= 3, kErrArg, "bad lmax or spin");
+    std::vector<double> th(theta, theta + npts);
+    auto p = std::make_unique<nusht_pointplan_s>();
+    p->pp = std::make_unique<PointPlan>(lmax, spin, th, S(s));
+    NUSHT_HIP(hipStreamSynchronize(S(s)));
+    *out = p.release();
+  });
+}
+
+int nusht_pointplan_destroy(nusht_pointplan_t p) {
+  delete p;
+  return NUSHT_OK;
+}
+
+int nusht_pointplan_synthesis(nusht_pointplan_t p, const double* alm, int ncomp, const double* phi,
+                              const double* gamma, double* out0, double* out1, nusht_stream_t s) {
+  return guard([&] {
+    NUSHT_REQUIRE(p, kErrArg, "null point plan");
+    p->pp->synthesis(reinterpret_cast<const double2*>(alm), ncomp, phi, gamma, out0, out1, S(s));
+  });
+}
+
 // ------------------------------------------------------------------ ring plan
 int nusht_ringplan_create(const nusht_ring_desc* d, nusht_stream_t s, nusht_ringplan_t* out) {
   return guard([&] {
@@ -115,8 +147,10 @@ int nusht_ringplan_create(const nusht_ring_desc* d, nusht_stream_t s, nusht_ring
     NUSHT_REQUIRE(d->spin >= 0 && d->spin <= 2, kErrArg, "spin must be 0, 1 or 2");
     std::vector<double> th(d->theta, d->theta + d->nring), p0(d->phi0, d->phi0 + d->nring);
     std::vector<long> nph(d->nph, d->nph + d->nring), ofs(d->ofs, d->ofs + d->nring);
+    std::vector<double> wt;
+    if (d->weight) wt.assign(d->weight, d->weight + d->nring);
     auto rp = std::make_unique<nusht_ringplan_s>();
-    rp->rp = std::make_unique<RingPlan>(d->lmax, d->spin, th, p0, nph, ofs, S(s));
+    rp->rp = std::make_unique<RingPlan>(d->lmax, d->spin, th, p0, nph, ofs, wt, S(s));
     NUSHT_HIP(hipStreamSynchronize(S(s)));
     *out = rp.release();
   });
@@ -145,20 +179,33 @@ int nusht_ringplan_get_info(nusht_ringplan_t p, nusht_ringplan_info* info) {
   });
 }
 
+static void check_ring_call(nusht_ringplan_t p, const void* map0, const void* map1, const void* alm, int ncomp) {
+  NUSHT_REQUIRE(p, kErrArg, "null ring plan");
+  NUSHT_REQUIRE(map0 && alm, kErrArg, "null map or alm pointer");
+  NUSHT_REQUIRE(p->rp->spin() == 0 || map1, kErrArg, "spin > 0 needs two maps (Q, U)");
+  NUSHT_REQUIRE(ncomp == 1 || (p->rp->spin() > 0 && ncomp == 2), kErrArg, "ncomp must be 1, or 2 for spin > 0");
+}
+
 int nusht_ringplan_synthesis(nusht_ringplan_t p, const double* alm, int ncomp, double* map0, double* map1,
                              nusht_stream_t s) {
   return guard([&] {
-    NUSHT_REQUIRE(p, kErrArg, "null ring plan");
-    NUSHT_REQUIRE(p->rp->spin() == 0 || map1, kErrArg, "spin > 0 needs two output maps");
+    check_ring_call(p, map0, map1, alm, ncomp);
     p->rp->synthesis(reinterpret_cast<const double2*>(alm), ncomp, map0, map1, S(s));
+  });
+}
+
+int nusht_ringplan_analysis(nusht_ringplan_t p, const double* map0, const double* map1, double* alm, int ncomp,
+                            nusht_stream_t s) {
+  return guard([&] {
+    check_ring_call(p, map0, map1, alm, ncomp);
+    p->rp->analysis(map0, map1, reinterpret_cast<double2*>(alm), ncomp, S(s));
   });
 }
 
 int nusht_ringplan_adjoint(nusht_ringplan_t p, const double* map0, const double* map1, double* alm, int ncomp,
                            nusht_stream_t s) {
   return guard([&] {
-    NUSHT_REQUIRE(p, kErrArg, "null ring plan");
-    NUSHT_REQUIRE(p->rp->spin() == 0 || map1, kErrArg, "spin > 0 needs two input maps");
+    check_ring_call(p, map0, map1, alm, ncomp);
     p->rp->adjoint(map0, map1, reinterpret_cast<double2*>(alm), ncomp, S(s));
   });
 }

@@ -19,8 +19,10 @@ enum class RingMethod : int { Direct = 0, Rader = 1, Bluestein = 2 };
 
 class RingPlan {
  public:
+  // weight: per-ring quadrature weights (may be empty; analysis() then fails)
   RingPlan(int lmax, int spin, const std::vector<double>& theta, const std::vector<double>& phi0,
-           const std::vector<long>& nph, const std::vector<long>& ofs, hipStream_t st);
+           const std::vector<long>& nph, const std::vector<long>& ofs, const std::vector<double>& weight,
+           hipStream_t st);
 
   int lmax() const { return lmax_; }
   int spin() const { return spin_; }
@@ -33,20 +35,24 @@ class RingPlan {
   void synthesis(const double2* alm, int ncomp, double* map0, double* map1, hipStream_t st);
   // exact adjoint
   void adjoint(const double* map0, const double* map1, double2* alm, int ncomp, hipStream_t st);
+  // adjoint of the map scaled by the ring weights (quadrature analysis: the inverse of synthesis
+  // for band-limited maps on a Gauss-Legendre grid)
+  void analysis(const double* map0, const double* map1, double2* alm, int ncomp, hipStream_t st);
+  bool has_weights() const { return weight_.bytes() != 0; }
   // spin-1 GRAD synthesis of dlm = sqrt(l(l+1)) dlm_scaled, then deflected angles (and gamma)
   void dlm2pointing(const double2* dlm_scaled, double* ptheta, double* pphi, double* gamma, hipStream_t st);
 
   size_t bytes() const;
 
  private:
-  void ring_pass(int mode, double* o0, double* o1, double* o2, hipStream_t st);
+  void ring_pass(int mode, double* o0, double* o1, double* o2, hipStream_t st, const double* wt = nullptr);
   int lmax_, spin_, nring_, nphi_;
   long npix_;
   RingMethod method_ = RingMethod::Direct;
   int mlen_ = 0;  // FFT length used inside the ring transform
   LegendreTables tab_;
   FftLength<double> fft_;
-  DevBuf theta_, phi0_, ofs_;
+  DevBuf theta_, phi0_, ofs_, weight_;
   DevBuf chirp_, kern_, perm_in_, perm_out_;  // Bluestein / Rader tables
   DevBuf sqll_;                               // sqrt(l(l+1))
   DevBuf G_;                                  // [nrows][nring] complex (m-major)

@@ -14,7 +14,8 @@
 namespace nusht {
 namespace {
 
-// 640 threads: a radix-13 stage over 8190 points (Rader for nphi = 8191) needs one butterfly per thread.
+// 768 threads: a radix-13 stage over 8190 points (Rader for nphi = 8191) needs one butterfly per thread,
+// and 768 rather than 640 threads halves its register spill.
 constexpr int kRingNT = 768;
 constexpr int kRingNMAX = 8320;
 
@@ -34,6 +35,7 @@ struct RingArgs {
   double* o2;
   const double* i0;
   const double* i1;
+  const double* wt;  // per-ring weights applied to the adjoint's input (null: none)
 };
 
 // In-place forward DFT of buf[0..n) (n = nphi) in LDS.
@@ -144,8 +146,9 @@ __global__ __launch_bounds__(kRingNT) void k_ring(RingArgs a) {
       }
     }
   } else {
+    const double w = a.wt ? a.wt[ring] : 1.0;
     for (int k = threadIdx.x; k < n; k += kRingNT)
-      buf[k] = double2{a.i0[ofs + k], (a.spin && a.i1) ? a.i1[ofs + k] : 0.0};
+      buf[k] = double2{w * a.i0[ofs + k], (a.spin && a.i1) ? w * a.i1[ofs + k] : 0.0};
     __syncthreads();
     ring_fwd_dft<METHOD>(buf, a);
     const int mlo = a.spin ? -L : 0;
@@ -213,11 +216,13 @@ __global__ void k_sqrt_ll(int lmax, double* out) {
 }  // namespace
 
 RingPlan::RingPlan(int lmax, int spin, const std::vector<double>& theta, const std::vector<double>& phi0,
-                   const std::vector<long>& nph, const std::vector<long>& ofs, hipStream_t st)
+                   const std::vector<long>& nph, const std::vector<long>& ofs, const std::vector<double>& weight,
+                   hipStream_t st)
     : lmax_(lmax), spin_(spin), nring_((int)theta.size()) {
   NUSHT_REQUIRE(nring_ >= 1, kErrArg, "ringplan: no rings");
   NUSHT_REQUIRE(phi0.size() == theta.size() && nph.size() == theta.size() && ofs.size() == theta.size(), kErrArg,
                 "ringplan: ring arrays differ in length");
+  NUSHT_REQUIRE(weight.empty() || weight.size() == theta.size(), kErrArg, "ringplan: one weight per ring");
   nphi_ = (int)nph[0];
   for (long v : nph)
     NUSHT_REQUIRE(v == nphi_, kErrUnsupported, "ringplan: rings must share one nphi");
@@ -288,6 +293,7 @@ RingPlan::RingPlan(int lmax, int spin, const std::vector<double>& theta, const s
   upload(theta_, theta);
   upload(phi0_, phi0);
   upload(ofs_, ofs);
+  if (!weight.empty()) upload(weight_, weight);
   tab_.tag = "ring";
   tab_.build(lmax, spin, theta, st);
   nrows_ = spin ? 2 * lmax + 1 : lmax + 1;
@@ -301,11 +307,11 @@ RingPlan::RingPlan(int lmax, int spin, const std::vector<double>& theta, const s
 }
 
 size_t RingPlan::bytes() const {
-  return tab_.bytes() + theta_.bytes() + phi0_.bytes() + ofs_.bytes() + chirp_.bytes() + kern_.bytes() +
+  return tab_.bytes() + theta_.bytes() + phi0_.bytes() + ofs_.bytes() + weight_.bytes() + chirp_.bytes() + kern_.bytes() +
          perm_in_.bytes() + perm_out_.bytes() + sqll_.bytes() + G_.bytes() + coef_.bytes();
 }
 
-void RingPlan::ring_pass(int mode, double* o0, double* o1, double* o2, hipStream_t st) {
+void RingPlan::ring_pass(int mode, double* o0, double* o1, double* o2, hipStream_t st, const double* wt) {
   RingArgs a;
   a.lmax = lmax_; a.spin = spin_; a.nphi = nphi_; a.nrows = nrows_; a.gstride = gstride_; a.method = (int)method_;
   a.mlen = mlen_;
@@ -317,6 +323,7 @@ void RingPlan::ring_pass(int mode, double* o0, double* o1, double* o2, hipStream
   a.perm_in = perm_in_.as<int>(); a.perm_out = perm_out_.as<int>();
   a.o0 = o0; a.o1 = o1; a.o2 = o2;
   a.i0 = o0; a.i1 = o1;
+  a.wt = wt;
   const size_t lds = sizeof(double2) * (size_t)std::max(nphi_, mlen_);
   static const char* names[4] = {"ring_dft_synth_s0", "ring_dft_synth_spin", "ring_dft_pointing", "ring_dft_adjoint"};
   StageScope sc(names[mode], st);
@@ -335,6 +342,12 @@ void RingPlan::synthesis(const double2* alm, int ncomp, double* map0, double* ma
 
 void RingPlan::adjoint(const double* map0, const double* map1, double2* alm, int ncomp, hipStream_t st) {
   ring_pass(3, const_cast<double*>(map0), const_cast<double*>(map1), nullptr, st);
+  legendre_adjoint(tab_, ncomp, G_.as<double2>(), gstride_, 1, alm, st);
+}
+
+void RingPlan::analysis(const double* map0, const double* map1, double2* alm, int ncomp, hipStream_t st) {
+  NUSHT_REQUIRE(has_weights(), kErrState, "ring plan created without ring weights: no analysis");
+  ring_pass(3, const_cast<double*>(map0), const_cast<double*>(map1), nullptr, st, weight_.as<double>());
   legendre_adjoint(tab_, ncomp, G_.as<double2>(), gstride_, 1, alm, st);
 }
 

@@ -58,7 +58,9 @@ class RingSHT:
         p0 = np.ascontiguousarray(geom.phi0, dtype=np.float64)
         nph = np.ascontiguousarray(geom.nph, dtype=np.int64)
         ofs = np.ascontiguousarray(geom.ofs, dtype=np.int64)
-        d = _lib.RingDesc(lmax, spin, th.size, th.ctypes.data, p0.ctypes.data, nph.ctypes.data, ofs.ctypes.data)
+        wt = np.ascontiguousarray(geom.weight, dtype=np.float64)
+        d = _lib.RingDesc(lmax, spin, th.size, th.ctypes.data, p0.ctypes.data, nph.ctypes.data, ofs.ctypes.data,
+                          wt.ctypes.data)
         h = ctypes.c_void_p()
         _lib.call("nusht_ringplan_create", ctypes.byref(d), _stream(), ctypes.byref(h))
         self._h = h
@@ -72,6 +74,37 @@ class RingSHT:
         h = getattr(self, "_h", None)
         if h is not None and h.value:
             _lib.load().nusht_ringplan_destroy(h)
+            self._h = None
+
+
+class PointPlan:
+    """Exact synthesis at a few points (colatitudes fixed at creation, host array)."""
+
+    def __init__(self, lmax: int, spin: int, theta):
+        th = np.ascontiguousarray(np.asarray(theta, dtype=np.float64).reshape(-1))
+        h = ctypes.c_void_p()
+        _lib.call("nusht_pointplan_create", lmax, spin, th.size, th.ctypes.data, _stream(), ctypes.byref(h))
+        self._h = h
+        self.lmax, self.spin, self.npts = lmax, spin, th.size
+
+    def synthesis(self, alm: torch.Tensor, ncomp: int, phi: torch.Tensor, gamma=None):
+        alm = _dev(alm, torch.complex128, "alm").contiguous()
+        phi = _dev(phi, torch.float64, "phi").contiguous()
+        if alm.numel() != ncomp * Alm.getsize(self.lmax, self.lmax):
+            raise ValueError(f"alm has {alm.numel()} entries, expected {ncomp} x nalm(lmax={self.lmax})")
+        if phi.numel() != self.npts or (gamma is not None and gamma.numel() != self.npts):
+            raise ValueError(f"phi / gamma must hold one value per point ({self.npts})")
+        if gamma is not None:
+            gamma = _dev(gamma, torch.float64, "gamma").contiguous()
+        out = torch.empty((2, self.npts), dtype=torch.float64, device="cuda")
+        _lib.call("nusht_pointplan_synthesis", self._h, _ptr(alm), ncomp, _ptr(phi), _ptr(gamma), _ptr(out[0]),
+                  _ptr(out[1] if self.spin else None), _stream())
+        return out[0] if self.spin == 0 else out
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            _lib.load().nusht_pointplan_destroy(h)
             self._h = None
 
 
@@ -257,6 +290,44 @@ class GPU_nusht_transformer:
                   _ptr(out[2] if gamma else None), _stream())
         return out
 
+    def gclm2lenpixs(self, gclm, mmax, spin, pixs, polrot=True, ptg=None, nthreads=None):
+        """Lensed values at the pointing pixels `pixs`, by exact (direct) synthesis at their deflected
+        positions (CPU_nufft_transformer.py:531-558).  Uses `ptg` if given, else the pointing of the last
+        gclm2lenmap / dlm2pointing call.  Returns a 1-D tensor (spin 0) or the tuple (Q, U)."""
+        assert spin >= 0, spin
+        gclm = _dev(gclm, torch.complex128, "gclm")
+        if ptg is None:
+            assert self.ptg is not None, "no pointing yet: pass ptg, or run gclm2lenmap / dlm2pointing first"
+            ptg = self.ptg
+        ptg = _dev(ptg, torch.float64, "ptg")
+        pixs = np.asarray(pixs, dtype=np.int64).reshape(-1)
+        if pixs.size == 0 or pixs.min() < 0 or pixs.max() >= ptg.shape[1]:
+            raise IndexError(f"pixs must be non-empty indices into the {ptg.shape[1]} pointing pixels")
+        pix = torch.as_tensor(pixs, device="cuda")
+        th = ptg[0].index_select(0, pix).contiguous()
+        ph = ptg[1].index_select(0, pix).contiguous()
+        gam = None
+        if spin and polrot:
+            assert self._gamma is not None, "polrot needs the rotation angle: compute the pointing from dlm_scaled"
+            gam = self._gamma.index_select(0, pix).contiguous()
+        plan = PointPlan(self.lmax, spin, th.cpu().numpy())
+        out = plan.synthesis(gclm, self._ncomp(gclm, spin), ph, gam)
+        torch.cuda.current_stream().synchronize()  # the plan is freed on return
+        return out if spin == 0 else (out[0], out[1])
+
+    def lensgclm(self, gclm, dlm, spin, lmax_out, nthreads=None, mmax=None, mmax_out=None, gclm_out=None,
+                 polrot=True, out_sht_mode="STANDARD"):
+        """Remapping alm -> lensed map -> adjoint synthesis on the pointing grid
+        (CPU_nufft_transformer.py:948-971).  `dlm` is the scaled deflection, as for gclm2lenmap.
+        out_sht_mode 'GRAD_ONLY' returns the gradient component only."""
+        assert lmax_out == self.lmax and (mmax_out is None or mmax_out == self.lmax), \
+            "lensgclm: lmax_out must equal the transformer's lmax"
+        if out_sht_mode not in ("STANDARD", "GRAD_ONLY"):
+            raise ValueError(f"unknown out_sht_mode {out_sht_mode}")
+        m = self.gclm2lenmap(gclm, self.lmax, self.lmax, dlm_scaled=dlm, spin=spin, polrot=polrot)
+        nc = 1 if (spin == 0 or out_sht_mode == "GRAD_ONLY") else 2
+        return self.adjoint_synthesis(m, lmax=self.lmax, mmax=self.lmax, out=gclm_out, spin=spin, ncomp=nc)
+
     def dlm2A(self, dlm, dclm=None, exact: bool = True):
         """Determinant of the magnification matrix of the deflection with gradient dlm (and curl dclm) on
         the pointing grid (cunusht/helper_GPU.py:113-161).  Note: dlm, not dlm_scaled.  See helpers.dlm2A."""
@@ -339,14 +410,17 @@ class GPU_nusht_transformer:
                   _ptr(outs[0]), _ptr(outs[1] if spin else None), _stream())
         return outs[0] if spin == 0 else outs
 
-    def adjoint_synthesis(self, synthmap, lmax=None, mmax=None, out=None, spin=0, ncomp=None):
-        """Exact adjoint of synthesis (GPU_nufft_transformer.py:369-383)."""
+    def adjoint_synthesis(self, synthmap, lmax=None, mmax=None, out=None, spin=0, ncomp=None, apply_weights=True):
+        """Adjoint synthesis on the pointing grid with the pixel weights applied, i.e. quadrature analysis
+        (GPU_nufft_transformer.py:372-383 -> SHTns analys, GPU_sht_transformer.py:65-73; the CPU backend's
+        ducc adjoint_synthesis with apply_weights, CPU_sht_transformer.py:37-48).  apply_weights=False gives
+        the exact transpose of synthesis."""
         self._check_lm(self.lmax if lmax is None else lmax, self.lmax if mmax is None else mmax)
         vals = self._values(synthmap, spin)
         nc = 1 if spin == 0 else (2 if ncomp is None else ncomp)
         res = self._alm_out(out, nc)
-        _lib.call("nusht_ringplan_adjoint", self._ring(spin)._h, _ptr(vals[0]), _ptr(vals[1]), _ptr(res), nc,
-                  _stream())
+        _lib.call("nusht_ringplan_analysis" if apply_weights else "nusht_ringplan_adjoint", self._ring(spin)._h,
+                  _ptr(vals[0]), _ptr(vals[1]), _ptr(res), nc, _stream())
         return res
 
     # ------------------------------------------------------------------ buffers

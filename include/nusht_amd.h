@@ -27,6 +27,7 @@ extern "C" {
 typedef void* nusht_stream_t;                 /* hipStream_t */
 typedef struct nusht_plan_s* nusht_plan_t;     /* nuSHT plan: one (lmax, spin, epsilon) */
 typedef struct nusht_ringplan_s* nusht_ringplan_t; /* SHT on the iso-latitude pointing grid */
+typedef struct nusht_pointplan_s* nusht_pointplan_t; /* exact synthesis at a few arbitrary points */
 
 #define NUSHT_OK 0
 #define NUSHT_ERR_ARG 1
@@ -80,10 +81,12 @@ int nusht_almxfl_c128(const double* alm, int lmax, int ncomp, const double* fl, 
 
 /* ---------------------------------------------------------------- ring plan (pointing grid SHT)
  * Replaces the GL-grid SHTns transformer of cunusht/sht/GPU_sht_transformer.py:28-34 and its
- * synthesis_cupy (:56-62), adjoint_synthesis_cupy (:65-73, exact transpose, no weights) and
- * synthesis_der1_cupy (:88-96) together with deflection.dlm2pointing
- * (cunusht/deflection/GPU_nufft_transformer.py:64-104).
- * theta, phi0, nph, ofs are HOST arrays of length nring; all rings must share one nph >= 2 lmax + 1. */
+ * synthesis_cupy (:56-62), adjoint_synthesis_cupy (:65-73: SHTns analys, the quadrature-weighted
+ * adjoint; with iw = 1 / weight the exact transpose) and synthesis_der1_cupy (:88-96) together with
+ * deflection.dlm2pointing (cunusht/deflection/GPU_nufft_transformer.py:64-104).
+ * theta, phi0, nph, ofs, weight are HOST arrays of length nring; all rings must share one
+ * nph >= 2 lmax + 1.  weight (per pixel of the ring, as Geom.weight of cunusht/geometry.py:29-57)
+ * may be NULL; nusht_ringplan_analysis then fails. */
 typedef struct {
   int lmax;
   int spin;             /* 0: scalar maps, 1: deflection / pointing, 2: polarisation */
@@ -92,6 +95,7 @@ typedef struct {
   const double* phi0;   /* host */
   const int64_t* nph;   /* host */
   const int64_t* ofs;   /* host */
+  const double* weight; /* host, may be NULL */
 } nusht_ring_desc;
 
 int nusht_ringplan_create(const nusht_ring_desc* desc, nusht_stream_t s, nusht_ringplan_t* out);
@@ -115,12 +119,28 @@ int nusht_legendre_variants(void);
 /* alm (ncomp x nalm) -> map0 (spin 0 map, or Q) and map1 (U; NULL for spin 0) */
 int nusht_ringplan_synthesis(nusht_ringplan_t p, const double* alm, int ncomp, double* map0, double* map1,
                              nusht_stream_t s);
-/* exact adjoint of the synthesis */
+/* quadrature analysis: the adjoint of the synthesis applied to map * weight
+ * (GPU_nufft_transformer.py:372-383 -> GPU_sht_transformer.py:65-73 SHTns analys;
+ * CPU_sht_transformer.py:37-48 ducc adjoint_synthesis with apply_weights) */
+int nusht_ringplan_analysis(nusht_ringplan_t p, const double* map0, const double* map1, double* alm, int ncomp,
+                            nusht_stream_t s);
+/* exact adjoint of the synthesis (no weights) */
 int nusht_ringplan_adjoint(nusht_ringplan_t p, const double* map0, const double* map1, double* alm, int ncomp,
                            nusht_stream_t s);
 /* spin-1 plans: dlm_scaled = dlm / sqrt(l(l+1)) -> deflected (theta, phi) and gamma (may be NULL) */
 int nusht_ringplan_dlm2pointing(nusht_ringplan_t p, const double* dlm_scaled, double* ptg_theta, double* ptg_phi,
                                 double* gamma, nusht_stream_t s);
+
+/* ---------------------------------------------------------------- point plan
+ * Exact synthesis (no nuFFT approximation) at arbitrary points, for few points.  Replaces the brute-force
+ * evaluation of gclm2lenpixs (cunusht/deflection/CPU_nufft_transformer.py:531-558).  theta: HOST array of
+ * npts colatitudes in [0, pi]; phi (and gamma, may be NULL) are device arrays given per call.  spin 0:
+ * out0 = f; spin s: (out0, out1) = (Q, U), rotated by exp(i s gamma) when gamma != NULL. */
+int nusht_pointplan_create(int lmax, int spin, int64_t npts, const double* theta, nusht_stream_t s,
+                           nusht_pointplan_t* out);
+int nusht_pointplan_destroy(nusht_pointplan_t p);
+int nusht_pointplan_synthesis(nusht_pointplan_t p, const double* alm, int ncomp, const double* phi,
+                              const double* gamma, double* out0, double* out1, nusht_stream_t s);
 
 /* ---------------------------------------------------------------- nuSHT plan
  * Replaces GPU_cufinufft_transformer.__init__ / plan (cunusht/deflection/GPU_nufft_transformer.py:108-173,

@@ -8,7 +8,9 @@ transforms, smallest 2^a 3^b 5^c >= n for real ones).
 Parity: good_size pinned against the sizes the reference logged
 (first_steps/quickstart.ipynb:209 ntheta=1584, nphi=3168 at lmax 1575; :239 GL nphi 3200).
 GL nodes/weights pinned against numpy.polynomial.legendre.leggauss and the exactness of
-the quadrature on polynomials of degree <= 2n-1.
+the quadrature on polynomials of degree <= 2n-1.  Pixel weights carry ducc0's factor
+2 pi / nphi (ducc0.misc.GL_weights, used at cunusht/geometry.py:280), pinned by their sum
+(4 pi) and by analysis(synthesis(alm)) = alm.
 """
 from __future__ import annotations
 
@@ -102,7 +104,7 @@ def gl_geometry(lmax: int, nphi: int | None = None) -> Rings:
     th, w = gl_nodes(n)
     nph = np.full(n, nphi, dtype=np.int64)
     ofs = np.arange(n, dtype=np.int64) * nphi
-    return Rings(th, np.zeros(n), nph, ofs, w / nph)
+    return Rings(th, np.zeros(n), nph, ofs, 2.0 * np.pi * w / nph)
 
 
 def cc_thetas(ntheta: int) -> np.ndarray:

@@ -1,0 +1,73 @@
+"""Pointing geometries: pixel weights and quadrature exactness (host logic, no GPU).
+
+The weights follow the ducc0 convention that cunusht/geometry.py:228-284 builds on (ring quadrature
+weight x 2 pi / nphi).  They are pinned here by what the quadrature rules fix: the total solid angle, exact
+integration of polynomials in cos(theta) up to each rule's degree, and analysis(synthesis(alm)) = alm on
+grids that resolve the band limit.
+"""
+import numpy as np
+import pytest
+
+from cunusht_amd import geometry as G
+from oracle import almutil, sht as osht
+from oracle import geometry as ogeom
+
+
+def _rule_exact(nodes_theta, w, degree):
+    x = np.cos(nodes_theta)
+    for k in range(degree + 1):
+        exact = 2.0 / (k + 1) if k % 2 == 0 else 0.0
+        assert abs(np.sum(w * x ** k) - exact) < 1e-13, (k, np.sum(w * x ** k), exact)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 8, 17, 64])
+def test_cc_weights_exact(n):
+    th = np.linspace(0.0, np.pi, n) if n > 1 else np.array([0.5 * np.pi])
+    _rule_exact(th, G.cc_weights(n), n - 1 if n > 1 else 1)
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 16, 33])
+def test_f1_weights_exact(n):
+    th = np.pi * (np.arange(n) + 0.5) / n
+    _rule_exact(th, G.f1_weights(n), n - 1 if n > 1 else 1)
+
+
+def test_cc_weights_known_values():
+    # Simpson's rule is Clenshaw-Curtis with three points: (1, 4, 1) / 3
+    assert np.allclose(G.cc_weights(3), [1 / 3, 4 / 3, 1 / 3], rtol=0, atol=1e-15)
+    # five points: 1/15, 8/15, 12/15, 8/15, 1/15
+    assert np.allclose(G.cc_weights(5), np.array([1, 8, 12, 8, 1]) / 15, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("geo", [("gl", {"lmax": 31}), ("gl", {"lmax": 20, "nphi": 45}),
+                                 ("cc", {"ntheta": 40, "nphi": 64}), ("f1", {"ntheta": 33, "nphi": 70})])
+def test_pixel_weights_sum_to_sphere(geo):
+    g = G.get_geom(geo)
+    w = np.repeat(g.weight, g.nph)
+    assert abs(np.sum(w) - 4 * np.pi) < 1e-12
+
+
+def test_gl_weights_match_oracle():
+    g = G.get_geom(("gl", {"lmax": 47}))
+    o = ogeom.gl_geometry(47)
+    assert np.allclose(g.theta, o.theta, rtol=0, atol=1e-15)
+    assert np.allclose(g.weight, o.weight, rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("spin", [0, 2])
+@pytest.mark.parametrize("geo", [("gl", {"lmax": 12}), ("cc", {"ntheta": 28, "nphi": 27}),
+                                 ("f1", {"ntheta": 27, "nphi": 26})])
+def test_weighted_adjoint_inverts_synthesis(spin, geo):
+    """Quadrature analysis (the adjoint applied to weight x map) returns the alm of a band-limited map:
+    GL with lmax + 1 rings, CC / F1 with more than 2 lmax + 1 rings (exact to degree n - 1 in cos(theta))."""
+    L = 12
+    rng = np.random.default_rng(spin)
+    nc = 1 if spin == 0 else 2
+    a = almutil.random_alm(L, spin, nc, rng)
+    ain = a[0] if spin == 0 else a
+    g = G.get_geom(geo)
+    th, ph = g.pixel_angles()
+    m = osht.synth_direct(ain, L, spin, th, ph)
+    w = np.repeat(g.weight, g.nph)
+    back = osht.adjoint_direct(w * m, L, spin, th, ph)
+    assert np.max(np.abs(np.reshape(back, (nc, -1)) - a)) < 1e-12
