@@ -46,7 +46,7 @@ class PlanInfo(ctypes.Structure):
 
 class RingInfo(ctypes.Structure):
     _fields_ = [("npix", _i64), ("nring", _i), ("ring_pairs", _i), ("ring_dft", _i), ("legendre_pair_steps", _i64),
-                ("legendre_variant", _i), ("legendre_ms", ctypes.c_float)]
+                ("legendre_variant", _i), ("legendre_ms", ctypes.c_float), ("ring_lengths", _i)]
 
 
 # name -> argtypes (all return int unless listed in _RESTYPES)

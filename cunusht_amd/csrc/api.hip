@@ -173,6 +173,7 @@ int nusht_ringplan_get_info(nusht_ringplan_t p, nusht_ringplan_info* info) {
     info->nring = p->rp->nring();
     info->ring_pairs = t.nslot;
     info->ring_dft = (int)p->rp->method();
+    info->ring_lengths = p->rp->nlengths();
     info->legendre_pair_steps = t.pair_steps;
     info->legendre_variant = t.variant;
     info->legendre_ms = t.tuned_ms;

@@ -1,21 +1,42 @@
-// SHT on iso-latitude rings with uniform phi sampling (the lensing/pointing grid), FP64.
+// SHT on iso-latitude rings with uniform phi sampling per ring (the lensing/pointing grid), FP64.
 //
 // Synthesis = Legendre ring coefficients G_m(theta_r) followed by one length-nph DFT per ring;
-// the adjoint runs the transposes.  Ring lengths fixed by the caller (nph = 2 lmax + 1 is
-// typical: 8191 is prime, 511 = 7 * 73, ...) are handled in LDS by a direct mixed-radix FFT
-// when nph factors into 2, 3, 5, 7, 11, 13, by Rader's algorithm when nph is prime with a
-// smooth nph - 1, and by Bluestein's chirp-z algorithm otherwise.  For spin 1 the deflected
-// pointing (theta', phi', gamma) is computed straight from the ring values in LDS, so the
-// deflection field itself is never written to HBM.
+// the adjoint runs the transposes.  Rings may differ in length (HEALPix, thinned Gauss-Legendre,
+// any ragged geometry); a ring shorter than 2 lmax + 1 receives the aliased sum of its G_m
+// (m folded modulo nph), as the exact sampling of the band-limited ring requires.  Every distinct
+// length gets its own in-LDS DFT: a direct mixed-radix FFT when nph factors into 2, 3, 5, 7, 13,
+// Rader's algorithm when nph is prime with a smooth nph - 1, Bluestein's chirp-z algorithm when
+// its padded length fits in LDS, and otherwise a split nph = r * b into r sub-transforms of
+// length b (direct or Bluestein) recombined with twiddles.  For spin 1 the deflected pointing
+// (theta', phi', gamma) is computed straight from the ring values in LDS, so the deflection field
+// itself is never written to HBM.
 #pragma once
 
 #include "common.h"
 #include "fft_engine.h"
 #include "legendre.h"
 
+#include <map>
+#include <memory>
+
 namespace nusht {
 
-enum class RingMethod : int { Direct = 0, Rader = 1, Bluestein = 2 };
+enum class RingMethod : int { Direct = 0, Rader = 1, Bluestein = 2, Split = 3 };
+constexpr int kRingMethods = 4;
+
+// One distinct ring length and how its DFT runs in LDS (kernel-argument view).
+struct RingLen {
+  int n;              // ring length
+  int method;         // RingMethod
+  int mlen;           // FFT length run in LDS: n (direct), n - 1 (Rader), padded (Bluestein, split sub)
+  int r, b;           // split: n = r * b, r sub-transforms of length b
+  int sub_bluestein;  // split: sub-transforms by Bluestein (else a direct FFT of length b)
+  FftDesc<double> fd; // of length mlen
+  const double2* chirp;  // Bluestein: e^{-i pi j^2 / len}, len = n (or b for split)
+  const double2* kern;   // Bluestein: FFT of the wrapped conjugate chirp (mlen); Rader: FFT of the permuted roots
+  const int* perm_in;    // Rader: g^q mod n
+  const int* perm_out;   // Rader: g^-q mod n
+};
 
 class RingPlan {
  public:
@@ -28,7 +49,8 @@ class RingPlan {
   int spin() const { return spin_; }
   long npix() const { return npix_; }
   int nring() const { return nring_; }
-  RingMethod method() const { return method_; }
+  int nlengths() const { return (int)lens_h_.size(); }
+  RingMethod method() const { return method_; }  // method of the longest ring
   const LegendreTables& tables() const { return tab_; }
 
   // alm (ncomp x nalm complex) -> map0 (spin 0: real map; spin s: Q) and map1 (U)
@@ -45,17 +67,30 @@ class RingPlan {
   size_t bytes() const;
 
  private:
+  struct LenTables {
+    RingLen d{};
+    DevBuf chirp, kern, perm_in, perm_out;
+  };
+  int add_length(int n, hipStream_t st);
+  const FftLength<double>& fft_of(int len);
   void ring_pass(int mode, double* o0, double* o1, double* o2, hipStream_t st, const double* wt = nullptr);
-  int lmax_, spin_, nring_, nphi_;
+
+  int lmax_, spin_, nring_;
   long npix_;
   RingMethod method_ = RingMethod::Direct;
-  int mlen_ = 0;  // FFT length used inside the ring transform
   LegendreTables tab_;
-  FftLength<double> fft_;
+  std::map<int, std::unique_ptr<FftLength<double>>> ffts_;  // by FFT length
+  std::vector<std::unique_ptr<LenTables>> lens_h_;
+  std::map<int, int> len_index_;                             // ring length -> index into lens_h_
+  DevBuf lens_;                                              // RingLen[nlengths]
+  DevBuf lenidx_;                                            // int[nring]
+  DevBuf rings_[kRingMethods];                               // ring ids per DFT method
+  int nrings_m_[kRingMethods] = {0, 0, 0, 0};
+  int uniform_[kRingMethods] = {-1, -1, -1, -1};             // the one length index of a method's rings, or -2
+  size_t lds_[kRingMethods] = {0, 0, 0, 0};
   DevBuf theta_, phi0_, ofs_, weight_;
-  DevBuf chirp_, kern_, perm_in_, perm_out_;  // Bluestein / Rader tables
-  DevBuf sqll_;                               // sqrt(l(l+1))
-  DevBuf G_;                                  // [nrows][nring] complex (m-major)
+  DevBuf sqll_;  // sqrt(l(l+1))
+  DevBuf G_;     // [nrows][nring] complex (m-major)
   DevBuf coef_;
   int nrows_ = 0;
   int gstride_ = 0;

@@ -84,9 +84,10 @@ int nusht_almxfl_c128(const double* alm, int lmax, int ncomp, const double* fl, 
  * synthesis_cupy (:56-62), adjoint_synthesis_cupy (:65-73: SHTns analys, the quadrature-weighted
  * adjoint; with iw = 1 / weight the exact transpose) and synthesis_der1_cupy (:88-96) together with
  * deflection.dlm2pointing (cunusht/deflection/GPU_nufft_transformer.py:64-104).
- * theta, phi0, nph, ofs, weight are HOST arrays of length nring; all rings must share one
- * nph >= 2 lmax + 1.  weight (per pixel of the ring, as Geom.weight of cunusht/geometry.py:29-57)
- * may be NULL; nusht_ringplan_analysis then fails. */
+ * theta, phi0, nph, ofs, weight are HOST arrays of length nring.  Rings may differ in length
+ * (1 <= nph <= 8320; HEALPix, thinned GL and other geometries of cunusht/geometry.py:194-297);
+ * rings shorter than 2 lmax + 1 are aliased as exact sampling requires.  weight (per pixel of the
+ * ring, as Geom.weight of cunusht/geometry.py:29-57) may be NULL; nusht_ringplan_analysis then fails. */
 typedef struct {
   int lmax;
   int spin;             /* 0: scalar maps, 1: deflection / pointing, 2: polarisation */
@@ -106,10 +107,11 @@ typedef struct {
   int64_t npix;
   int nring;
   int ring_pairs;                /* rings after north/south pairing */
-  int ring_dft;                  /* 0 direct mixed-radix, 1 Rader, 2 Bluestein */
+  int ring_dft;                  /* DFT of the longest ring: 0 direct mixed-radix, 1 Rader, 2 Bluestein, 3 split */
   int64_t legendre_pair_steps;   /* (l, ring pair) recursion steps one synthesis runs */
   int legendre_variant;          /* synthesis kernel variant chosen by the plan-time autotuner */
   float legendre_ms;             /* its measured time for one map */
+  int ring_lengths;              /* distinct ring lengths (one DFT description each) */
 } nusht_ringplan_info;
 /* Plan properties (no counterpart in the reference; reported for measurement). */
 int nusht_ringplan_get_info(nusht_ringplan_t p, nusht_ringplan_info* info);

@@ -71,3 +71,61 @@ def test_weighted_adjoint_inverts_synthesis(spin, geo):
     w = np.repeat(g.weight, g.nph)
     back = osht.adjoint_direct(w * m, L, spin, th, ph)
     assert np.max(np.abs(np.reshape(back, (nc, -1)) - a)) < 1e-12
+
+
+# ------------------------------------------------------------------ HEALPix, thinned GL, helpers
+def test_healpix_nside1_known_pixels():
+    """HEALPix nside 1: 3 rings of 4 pixels at z = 2/3, 0, -2/3; pixel 0 at phi = pi/4, pixel 4 at phi = 0."""
+    g = G.get_geom(("healpix", {"nside": 1}))
+    assert np.allclose(np.cos(g.theta), [2 / 3, 0, -2 / 3], atol=1e-15)
+    assert list(g.nph) == [4, 4, 4] and list(g.ofs) == [0, 4, 8]
+    assert np.allclose(g.phi0, [np.pi / 4, 0.0, np.pi / 4], atol=1e-15)
+
+
+@pytest.mark.parametrize("nside", [2, 8, 32])
+def test_healpix_rings(nside):
+    g = G.get_geom(("healpix", {"nside": nside}))
+    assert g.npix() == 12 * nside ** 2 and g.theta.size == 4 * nside - 1
+    assert abs(g.fsky() - 1.0) < 1e-14
+    # ring z values of the polar cap, the cap / belt seam (z = 2/3) and the equator; north-south mirror
+    z = np.cos(g.theta)
+    i = np.arange(1, nside)
+    assert np.allclose(z[:nside - 1], 1 - i ** 2 / (3.0 * nside ** 2), rtol=0, atol=1e-14)
+    assert abs(z[nside - 1] - 2 / 3) < 1e-14 and abs(z[2 * nside - 1]) < 1e-15
+    assert np.allclose(g.theta + g.theta[::-1], np.pi, atol=1e-14)
+    assert list(g.nph[:nside]) == list(4 * np.arange(1, nside + 1))
+    # the polar rings are exact even at tiny colatitude
+    assert abs(g.theta[0] - 2 * np.arcsin(1 / (np.sqrt(6) * nside))) == 0
+
+
+def test_thingauss_matches_rule_and_thins():
+    g = G.get_geom(("tgl", {"lmax": 300, "smax": 2}))
+    full = G.get_geom(("gl", {"lmax": 300}))
+    assert np.array_equal(g.theta, full.theta)
+    assert abs(g.fsky() - 1.0) < 1e-13
+    assert g.nph.min() < full.nph.min() and g.nph.max() <= full.nph.max()
+    m = np.maximum(G.st2mmax(2, g.theta, 300), G.st2mmax(-2, g.theta, 300))
+    assert np.all(g.nph >= np.minimum(2 * m + 1, 2 * 300 + 1) - 1e-9)
+
+
+def test_restrict_split_thinout():
+    g = G.get_geom(("gl", {"lmax": 40}))
+    bands = g.split(3)
+    assert sum(b.npix() for b in bands) == g.npix()
+    pix = np.sort(np.concatenate([G.Geom.rings2pix(b, np.arange(b.theta.size)) for b in bands]))
+    assert np.array_equal(pix, np.arange(g.npix()))
+    r = g.restrict(0.0, 0.5, False, update_ringstart=True)
+    assert np.all(r.theta <= 0.5) and r.ofs[0] == 0
+    t = g.thinout(0)
+    assert abs(t.fsky() - 1.0) < 1e-13 and t.npix() <= g.npix()
+    phs = G.Geom.phis(g, 3)
+    assert phs.size == g.nph[3] and np.all((phs >= 0) & (phs < 2 * np.pi))
+    assert "healpix" in G.Geom.get_supported_geometries()
+
+
+def test_pbounds():
+    pb = G.pbounds(0.1, 0.4)
+    assert pb.contains(np.array([0.0, 0.29, 2 * np.pi - 0.09])).all()
+    assert not pb.contains(np.array([0.31, np.pi])).any()
+    assert pb.get_range() == pytest.approx(0.4) and pb == G.pbounds(0.1, 0.4)
+    assert G.pbdGeometry(G.get_geom(("healpix", {"nside": 2})), pb).npix() == 48

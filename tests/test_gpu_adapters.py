@@ -157,3 +157,131 @@ def test_lensgclm_zero_deflection_is_identity():
     zero = np.zeros_like(g[0])
     got = T.lensgclm(_t(g), _t(zero), 2, L).cpu().numpy()
     assert np.max(np.abs(got - g)) < 1e-9
+
+
+# ------------------------------------------------------------------ ragged ring geometries
+def _ring_plan_calls(geom, L, spin, alm, x, weighted):
+    """synthesis of alm and (weighted) adjoint of x on `geom` through the C ABI ring plan."""
+    import ctypes
+    from cunusht_amd import _lib
+    from cunusht_amd.transformer import RingSHT
+    rs = RingSHT(geom, L, spin)
+    nc = alm.shape[0] if alm.ndim == 2 else 1
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    p = lambda t: ctypes.c_void_p(0 if t is None else t.data_ptr())
+    a = _t(alm)
+    out = torch.empty((2, geom.npix()), dtype=torch.float64, device="cuda")
+    _lib.call("nusht_ringplan_synthesis", rs._h, p(a), nc, p(out[0]), p(out[1] if spin else None), s)
+    xd = _t(x)
+    res = torch.empty((nc, almutil.nalm(L)), dtype=torch.complex128, device="cuda")
+    _lib.call("nusht_ringplan_analysis" if weighted else "nusht_ringplan_adjoint", rs._h, p(xd[0]),
+              p(xd[1] if spin else None), p(res), nc, s)
+    torch.cuda.synchronize()
+    info = rs.info()
+    return (out[0].cpu().numpy() if spin == 0 else out.cpu().numpy()), res.cpu().numpy(), info
+
+
+@pytest.mark.parametrize("spin", [0, 2])
+def test_ragged_rings_every_dft_method(spin):
+    """Rings of different lengths, shorter (aliased) and longer than 2 lmax + 1, through every ring DFT:
+    direct (1, 2, 3, 64, 96), Rader (31, 97, 8191), Bluestein (22, 89), split (4124 = 2 x 2062 with Bluestein
+    sub-transforms, 8189 = 19 x 431).  Synthesis and the unweighted adjoint vs direct summation."""
+    from cunusht_amd.geometry import Geom
+    L = 40
+    rng = np.random.default_rng(30 + spin)
+    nph = np.array([96, 1, 64, 31, 4124, 22, 2, 97, 89, 3, 8191, 8189], dtype=np.int64)
+    th = np.sort(rng.uniform(0.05, np.pi - 0.05, nph.size))
+    th[1] = np.pi - th[0]  # one mirrored pair shares a recursion
+    phi0 = rng.uniform(0, 0.3, nph.size)
+    ofs = np.concatenate([[0], np.cumsum(nph[:-1])])
+    g = Geom(th, phi0, nph, ofs, np.ones(nph.size))
+    nc = 1 if spin == 0 else 2
+    a = almutil.random_alm(L, spin, nc, rng)
+    x = rng.standard_normal((2, g.npix()))
+    got, ad, info = _ring_plan_calls(g, L, spin, a, x, weighted=False)
+    assert info["ring_lengths"] == nph.size and info["ring_dft"] == 1  # the longest ring (8191) is Rader
+    tt, pp = g.pixel_angles()
+    ain = a[0] if spin == 0 else a
+    ref = osht.synth_direct(ain, L, spin, tt, pp)
+    if spin:
+        ref = np.array([ref.real, ref.imag])
+    assert _relmax(got, ref) < 1e-12
+    xv = x[0] if spin == 0 else x[0] + 1j * x[1]
+    refad = np.reshape(osht.adjoint_direct(xv, L, spin, tt, pp), (nc, -1))
+    assert _relmax(ad, refad) < 1e-12
+
+
+def test_unsupported_ring_length_is_rejected():
+    """4177 is prime with 4176 = 2^4 3^2 29: no in-LDS DFT; the plan refuses it on the host."""
+    from cunusht_amd._lib import NuShtError
+    from cunusht_amd.geometry import Geom
+    from cunusht_amd.transformer import RingSHT
+    g = Geom(np.array([1.0]), np.zeros(1), np.array([4177]), np.zeros(1, dtype=np.int64), np.ones(1))
+    with pytest.raises(NuShtError, match="not supported"):
+        RingSHT(g, 8, 0)
+
+
+@pytest.mark.parametrize("spin", [0, 1, 2])
+def test_healpix_synthesis_and_weighted_adjoint(spin):
+    """HEALPix nside 16 (polar rings of 4 .. 60 pixels, aliased for lmax 40; phi0 offsets) vs direct summation."""
+    import cunusht_amd
+    L, nside = 40, 16
+    rng = np.random.default_rng(50 + spin)
+    T = cunusht_amd.get_transformer("GPU")(("healpix", {"nside": nside, "lmax": L}))
+    nc = 1 if spin == 0 else 2
+    a = almutil.random_alm(L, spin, nc, rng)
+    ain = a[0] if spin == 0 else a
+    got = T.synthesis(_t(ain), lmax=L, mmax=L, spin=spin).cpu().numpy()
+    g = T.geom
+    tt, pp = g.pixel_angles()
+    ref = osht.synth_direct(ain, L, spin, tt, pp)
+    if spin:
+        ref = np.array([ref.real, ref.imag])
+    assert _relmax(got, ref) < 1e-12
+    x = rng.standard_normal((nc, g.npix())) if spin else rng.standard_normal(g.npix())
+    ad = T.adjoint_synthesis(_t(x), lmax=L, mmax=L, spin=spin).cpu().numpy().reshape(nc, -1)
+    xv = x if spin == 0 else x[0] + 1j * x[1]
+    w = np.repeat(g.weight, g.nph)
+    refad = np.reshape(osht.adjoint_direct(w * xv, L, spin, tt, pp), (nc, -1))
+    assert _relmax(ad, refad) < 1e-12
+
+
+@pytest.mark.parametrize("spin", [0, 2])
+def test_thinned_gl_analysis_inverts_synthesis(spin):
+    """Thinned GL at lmax 400: polar rings carry fewer than 2 lmax + 1 pixels; the aliased m are below the
+    polar-optimisation threshold, so quadrature analysis still returns the alm."""
+    import cunusht_amd
+    L = 400
+    rng = np.random.default_rng(60 + spin)
+    T = cunusht_amd.get_transformer("GPU")(("tgl", {"lmax": L, "smax": 2}))
+    assert T.geom.nph.min() < 2 * L + 1
+    nc = 1 if spin == 0 else 2
+    a = almutil.random_alm(L, spin, nc, rng)
+    ain = a[0] if spin == 0 else a
+    m = T.synthesis(_t(ain), lmax=L, mmax=L, spin=spin)
+    back = T.adjoint_synthesis(m, lmax=L, mmax=L, spin=spin).cpu().numpy().reshape(nc, -1)
+    assert np.max(np.abs(back - a)) < 1e-10
+
+
+def test_healpix_pointing_and_lensing():
+    """dlm2pointing on HEALPix rings vs the oracle, and gclm2lenmap on that pointing vs the oracle pipeline."""
+    import cunusht_amd
+    from oracle import pointing as opointing
+    L, nside = 32, 16
+    rng = np.random.default_rng(70)
+    dlm = _deflection(L, rng, 1e-3)
+    T = cunusht_amd.get_transformer("GPU")(("healpix", {"nside": nside, "lmax": L}), nuFFTtype=2, epsilon=1e-12)
+    pg = T.dlm2pointing(_t(dlm), gamma=True).cpu().numpy()
+    g = T.geom
+    tt, pp = g.pixel_angles()
+    ll = np.arange(L + 1)
+    d = osht.synth_direct(np.array([almutil.almxfl(dlm, np.sqrt(ll * (ll + 1.0)), L), np.zeros_like(dlm)]), L, 1,
+                          tt, pp)
+    tp, ph = opointing.deflected_angles(tt, pp, d.real, d.imag)
+    assert np.max(np.abs(pg[0] - tp)) < 1e-12
+    dphi = (pg[1] - ph + np.pi) % (2 * np.pi) - np.pi
+    assert np.max(np.abs(dphi)) < 1e-12
+    gl = almutil.random_alm(L, 0, 1, rng)[0]
+    got = T.gclm2lenmap(_t(gl), L, L, dlm_scaled=_t(dlm)).cpu().numpy()
+    ref = osht.synth_direct(gl, L, 0, tp, ph)
+    assert _relmax(got, ref) < 1e-9
