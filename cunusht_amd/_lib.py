@@ -35,7 +35,16 @@ class RingDesc(ctypes.Structure):
 
 class PlanDesc(ctypes.Structure):
     _fields_ = [("lmax", _i), ("mmax", _i), ("spin", _i), ("ncomp", _i), ("epsilon", _d), ("npts_max", _i64),
-                ("sigma", _d)]
+                ("sigma", _d), ("max_maps", _i)]
+
+
+JOB_GCLM2LENMAP, JOB_LENMAP2GCLM, JOB_NUSHT2D2, JOB_NUSHT2D1 = 0, 1, 2, 3
+
+
+class Job(ctypes.Structure):
+    _fields_ = [("kind", _i), ("plan", _vp), ("ring", _vp), ("dlm_scaled", _vp), ("polrot", _i), ("gamma", _vp),
+                ("alm_in", _vp), ("alm_out", _vp), ("map_in0", _vp), ("map_in1", _vp), ("map_out0", _vp),
+                ("map_out1", _vp)]
 
 
 class PlanInfo(ctypes.Structure):
@@ -46,7 +55,8 @@ class PlanInfo(ctypes.Structure):
 
 class RingInfo(ctypes.Structure):
     _fields_ = [("npix", _i64), ("nring", _i), ("ring_pairs", _i), ("ring_dft", _i), ("legendre_pair_steps", _i64),
-                ("legendre_variant", _i), ("legendre_ms", ctypes.c_float), ("ring_lengths", _i)]
+                ("legendre_variant", _i), ("legendre_ms", ctypes.c_float), ("ring_lengths", _i),
+                ("workspace_bytes", _i64)]
 
 
 # name -> argtypes (all return int unless listed in _RESTYPES)
@@ -77,6 +87,8 @@ SIGNATURES = {
     "nusht_plan_create": [ctypes.POINTER(PlanDesc), _vp, ctypes.POINTER(_vp)],
     "nusht_plan_destroy": [_vp],
     "nusht_plan_get_info": [_vp, ctypes.POINTER(PlanInfo)],
+    "nusht_plan_workspace_bytes": [_vp],
+    "nusht_batch_execute": [ctypes.POINTER(Job), _i, _vp],
     "nusht_plan_set_pts": [_vp, _vp, _vp, _i64, _vp],
     "nusht_plan_set_pts_async": [_vp, _vp, _vp, _i64, _vp],
     "nusht_nusht2d2": [_vp, _vp, _vp, _vp, _vp, _vp],
@@ -89,7 +101,7 @@ SIGNATURES = {
     "nusht_timing_count": [],
     "nusht_timing_get": [_i, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(_d), ctypes.POINTER(_i64)],
 }
-_RESTYPES = {"nusht_last_error": ctypes.c_char_p, "nusht_ringplan_npix": _i64}
+_RESTYPES = {"nusht_last_error": ctypes.c_char_p, "nusht_ringplan_npix": _i64, "nusht_plan_workspace_bytes": _i64}
 
 _LIB = None
 

@@ -1,9 +1,13 @@
 // extern "C" boundary of libnusht_amd.so (declared in include/nusht_amd.h).
 #include "../../include/nusht_amd.h"
 
+#include <algorithm>
 #include <exception>
 #include <memory>
+#include <mutex>
 #include <new>
+#include <unordered_map>
+#include <vector>
 
 #include "legendre.h"
 #include "nufft.h"
@@ -22,7 +26,9 @@ struct nusht_plan_s {
   std::unique_ptr<NufftPlan> nu;
   DevBuf coef, G, ptg;
   int nmrows = 0;
+  int max_maps = kMaxMaps;  // maps per batched synthesis the G buffer holds
   static constexpr int kMaxMaps = 8;
+  size_t bytes() const { return tab.bytes() + nu->bytes() + coef.bytes() + G.bytes() + ptg.bytes(); }
 };
 
 struct nusht_ringplan_s {
@@ -174,6 +180,7 @@ int nusht_ringplan_get_info(nusht_ringplan_t p, nusht_ringplan_info* info) {
     info->ring_pairs = t.nslot;
     info->ring_dft = (int)p->rp->method();
     info->ring_lengths = p->rp->nlengths();
+    info->workspace_bytes = (int64_t)p->rp->bytes();
     info->legendre_pair_steps = t.pair_steps;
     info->legendre_variant = t.variant;
     info->legendre_ms = t.tuned_ms;
@@ -237,9 +244,11 @@ int nusht_plan_create(const nusht_plan_desc* d, nusht_stream_t s, nusht_plan_t* 
     p->tab.tag = "cc";
     p->tab.build(d->lmax, d->spin, th, S(s));
     p->nu = std::make_unique<NufftPlan>(p->np, p->npts_max, S(s));
+    NUSHT_REQUIRE(d->max_maps >= 0 && d->max_maps <= nusht_plan_s::kMaxMaps, kErrArg, "max_maps must be in [0, 8]");
+    p->max_maps = d->max_maps ? d->max_maps : nusht_plan_s::kMaxMaps;
     p->nmrows = d->spin ? 2 * d->lmax + 1 : d->lmax + 1;
-    p->coef.alloc(legendre_coef_bytes(p->tab, nusht_plan_s::kMaxMaps));
-    p->G.alloc(sizeof(double2) * (size_t)p->nmrows * p->np.ntheta * nusht_plan_s::kMaxMaps);
+    p->coef.alloc(legendre_coef_bytes(p->tab, p->max_maps));
+    p->G.alloc(sizeof(double2) * (size_t)p->nmrows * p->np.ntheta * p->max_maps);
     p->ptg.alloc(sizeof(double) * 3 * (size_t)p->npts_max);
     legendre_autotune(p->tab, p->ncomp, p->coef.as<double2>(), p->G.as<double2>(), p->np.ntheta, 1, S(s));
     NUSHT_HIP(hipStreamSynchronize(S(s)));
@@ -263,12 +272,14 @@ int nusht_plan_get_info(nusht_plan_t p, nusht_plan_info* info) {
     info->single_precision = p->np.single ? 1 : 0;
     info->sigma = p->np.sigma;
     info->beta = p->np.beta;
-    info->workspace_bytes = (int64_t)(p->tab.bytes() + p->nu->bytes() + p->coef.bytes() + p->G.bytes() + p->ptg.bytes());
+    info->workspace_bytes = (int64_t)p->bytes();
     info->legendre_pair_steps = p->tab.pair_steps;
     info->legendre_variant = p->tab.variant;
     info->legendre_ms = p->tab.tuned_ms;
   });
 }
+
+int64_t nusht_plan_workspace_bytes(nusht_plan_t p) { return p ? (int64_t)p->bytes() : -1; }
 
 int nusht_plan_set_pts(nusht_plan_t p, const double* theta, const double* phi, int64_t npts, nusht_stream_t s) {
   return guard([&] {
@@ -295,13 +306,59 @@ static void synth_batch(nusht_plan_t p, int nmaps, const double* alm, long alm_s
                  out1 ? out1 + b * out_stride : nullptr, st);
 }
 
+// ------------------------------------------------------------------ composed operations (no guard; shared by
+// the single-call entry points and the batch executor)
+namespace {
+
+void op_nusht2d2(nusht_plan_t p, const double* alm, double* out0, double* out1, const double* gamma, hipStream_t st) {
+  synth_batch(p, 1, alm, 0, out0, out1, 0, gamma, st);
+}
+
+void op_nusht2d1(nusht_plan_t p, const double* in0, const double* in1, const double* gamma, double* alm,
+                 hipStream_t st) {
+  p->nu->type1(in0, in1, gamma, p->G.as<double2>(), p->np.ntheta, st);
+  legendre_adjoint(p->tab, p->ncomp, p->G.as<double2>(), p->np.ntheta, 1, reinterpret_cast<double2*>(alm), st);
+}
+
+void check_lensing(nusht_plan_t p, nusht_ringplan_t ptg) {
+  NUSHT_REQUIRE(ptg && ptg->rp->spin() == 1, kErrArg, "need a spin-1 ring plan for the pointing");
+  NUSHT_REQUIRE(ptg->rp->lmax() == p->lmax, kErrArg, "ring plan lmax differs from the nuSHT plan");
+  NUSHT_REQUIRE(ptg->rp->npix() <= p->npts_max, kErrArg, "pointing grid larger than the plan's npts_max");
+}
+
+// pointing of the ring plan's grid into the plan's point buffers; returns gamma (or null)
+const double* lensing_points(nusht_plan_t p, nusht_ringplan_t ptg, const double* dlm_scaled, int polrot,
+                             hipStream_t st) {
+  double* th = p->ptg.as<double>();
+  double* ph = th + p->npts_max;
+  double* ga = ph + p->npts_max;
+  const bool rot = polrot && p->spin > 0;
+  ptg->rp->dlm2pointing(reinterpret_cast<const double2*>(dlm_scaled), th, ph, rot ? ga : nullptr, st);
+  p->nu->set_pts(th, ph, ptg->rp->npix(), st, /*check=*/false);  // pointing from the ring plan
+  return rot ? ga : nullptr;
+}
+
+void op_gclm2lenmap(nusht_plan_t p, nusht_ringplan_t ptg, const double* gclm, const double* dlm_scaled,
+                    double* lenmap0, double* lenmap1, int polrot, hipStream_t st) {
+  const double* ga = lensing_points(p, ptg, dlm_scaled, polrot, st);
+  synth_batch(p, 1, gclm, 0, lenmap0, lenmap1, 0, ga, st);
+}
+
+void op_lenmap2gclm(nusht_plan_t p, nusht_ringplan_t ptg, const double* lenmap0, const double* lenmap1,
+                    const double* dlm_scaled, double* gclm, int polrot, hipStream_t st) {
+  const double* ga = lensing_points(p, ptg, dlm_scaled, polrot, st);
+  op_nusht2d1(p, lenmap0, lenmap1, ga, gclm, st);
+}
+
+}  // namespace
+
 int nusht_nusht2d2(nusht_plan_t p, const double* alm, double* out0, double* out1, const double* gamma,
                    nusht_stream_t s) {
   return guard([&] {
     check_plan(p);
     NUSHT_REQUIRE(alm && out0, kErrArg, "null array");
     NUSHT_REQUIRE(p->spin == 0 || out1, kErrArg, "spin > 0 needs two output arrays");
-    synth_batch(p, 1, alm, 0, out0, out1, 0, gamma, S(s));
+    op_nusht2d2(p, alm, out0, out1, gamma, S(s));
   });
 }
 
@@ -309,7 +366,8 @@ int nusht_nusht2d2_batch(nusht_plan_t p, int nmaps, const double* alm, int64_t a
                          double* out1, int64_t out_stride, const double* gamma, nusht_stream_t s) {
   return guard([&] {
     check_plan(p);
-    NUSHT_REQUIRE(nmaps >= 1 && nmaps <= nusht_plan_s::kMaxMaps, kErrArg, "nmaps must be in [1, 8]");
+    NUSHT_REQUIRE(nmaps >= 1 && nmaps <= p->max_maps, kErrArg,
+                  "nmaps must be in [1, " + std::to_string(p->max_maps) + "] (the plan's max_maps)");
     NUSHT_REQUIRE(alm && out0, kErrArg, "null array");
     NUSHT_REQUIRE(p->spin == 0 || out1, kErrArg, "spin > 0 needs two output arrays");
     synth_batch(p, nmaps, alm, alm_stride, out0, out1, out_stride, gamma, S(s));
@@ -322,8 +380,7 @@ int nusht_nusht2d1(nusht_plan_t p, const double* in0, const double* in1, const d
     check_plan(p);
     NUSHT_REQUIRE(in0 && alm, kErrArg, "null array");
     NUSHT_REQUIRE(p->spin == 0 || in1, kErrArg, "spin > 0 needs two input arrays");
-    p->nu->type1(in0, in1, gamma, p->G.as<double2>(), p->np.ntheta, S(s));
-    legendre_adjoint(p->tab, p->ncomp, p->G.as<double2>(), p->np.ntheta, 1, reinterpret_cast<double2*>(alm), S(s));
+    op_nusht2d1(p, in0, in1, gamma, alm, S(s));
   });
 }
 
@@ -331,17 +388,10 @@ int nusht_gclm2lenmap(nusht_plan_t p, nusht_ringplan_t ptg, const double* gclm, 
                       double* lenmap0, double* lenmap1, int polrot, nusht_stream_t s) {
   return guard([&] {
     check_plan(p);
-    NUSHT_REQUIRE(ptg && ptg->rp->spin() == 1, kErrArg, "need a spin-1 ring plan for the pointing");
-    NUSHT_REQUIRE(ptg->rp->lmax() == p->lmax, kErrArg, "ring plan lmax differs from the nuSHT plan");
-    const long npix = ptg->rp->npix();
-    NUSHT_REQUIRE(npix <= p->npts_max, kErrArg, "pointing grid larger than the plan's npts_max");
-    double* th = p->ptg.as<double>();
-    double* ph = th + p->npts_max;
-    double* ga = ph + p->npts_max;
-    const bool rot = polrot && p->spin > 0;
-    ptg->rp->dlm2pointing(reinterpret_cast<const double2*>(dlm_scaled), th, ph, rot ? ga : nullptr, S(s));
-    p->nu->set_pts(th, ph, npix, S(s), /*check=*/false);  // pointing from the ring plan
-    synth_batch(p, 1, gclm, 0, lenmap0, lenmap1, 0, rot ? ga : nullptr, S(s));
+    check_lensing(p, ptg);
+    NUSHT_REQUIRE(gclm && dlm_scaled && lenmap0, kErrArg, "null array");
+    NUSHT_REQUIRE(p->spin == 0 || lenmap1, kErrArg, "spin > 0 needs two output arrays");
+    op_gclm2lenmap(p, ptg, gclm, dlm_scaled, lenmap0, lenmap1, polrot, S(s));
   });
 }
 
@@ -349,18 +399,142 @@ int nusht_lenmap2gclm(nusht_plan_t p, nusht_ringplan_t ptg, const double* lenmap
                       const double* dlm_scaled, double* gclm, int polrot, nusht_stream_t s) {
   return guard([&] {
     check_plan(p);
-    NUSHT_REQUIRE(ptg && ptg->rp->spin() == 1, kErrArg, "need a spin-1 ring plan for the pointing");
-    NUSHT_REQUIRE(ptg->rp->lmax() == p->lmax, kErrArg, "ring plan lmax differs from the nuSHT plan");
-    const long npix = ptg->rp->npix();
-    NUSHT_REQUIRE(npix <= p->npts_max, kErrArg, "pointing grid larger than the plan's npts_max");
-    double* th = p->ptg.as<double>();
-    double* ph = th + p->npts_max;
-    double* ga = ph + p->npts_max;
-    const bool rot = polrot && p->spin > 0;
-    ptg->rp->dlm2pointing(reinterpret_cast<const double2*>(dlm_scaled), th, ph, rot ? ga : nullptr, S(s));
-    p->nu->set_pts(th, ph, npix, S(s), /*check=*/false);  // pointing from the ring plan
-    p->nu->type1(lenmap0, lenmap1, rot ? ga : nullptr, p->G.as<double2>(), p->np.ntheta, S(s));
-    legendre_adjoint(p->tab, p->ncomp, p->G.as<double2>(), p->np.ntheta, 1, reinterpret_cast<double2*>(gclm), S(s));
+    check_lensing(p, ptg);
+    NUSHT_REQUIRE(lenmap0 && dlm_scaled && gclm, kErrArg, "null array");
+    NUSHT_REQUIRE(p->spin == 0 || lenmap1, kErrArg, "spin > 0 needs two input arrays");
+    op_lenmap2gclm(p, ptg, lenmap0, lenmap1, dlm_scaled, gclm, polrot, S(s));
+  });
+}
+
+// ------------------------------------------------------------------ batches of independent transforms
+namespace {
+
+// Internal streams of one device for the batch executor, created on first use and kept for the life
+// of the process (default priority, non-blocking with respect to the legacy stream).
+struct BatchStreams {
+  static constexpr int K = 4;
+  hipStream_t s[K] = {};
+  hipEvent_t fork = nullptr;
+  hipEvent_t join[K] = {};
+  bool ready = false;
+  std::mutex mu;
+};
+
+BatchStreams& batch_streams() {
+  static BatchStreams pools[64];
+  int dev = 0;
+  NUSHT_HIP(hipGetDevice(&dev));
+  NUSHT_REQUIRE(dev >= 0 && dev < 64, kErrUnsupported, "batch: device index out of range");
+  return pools[dev];
+}
+
+void check_job(const nusht_job& j, int i) {
+  const std::string at = "batch job " + std::to_string(i) + ": ";
+  NUSHT_REQUIRE(j.plan, kErrArg, at + "null plan");
+  nusht_plan_t p = j.plan;
+  switch (j.kind) {
+    case NUSHT_JOB_GCLM2LENMAP:
+    case NUSHT_JOB_LENMAP2GCLM:
+      check_lensing(p, j.ring);
+      NUSHT_REQUIRE(j.dlm_scaled, kErrArg, at + "null dlm_scaled");
+      break;
+    case NUSHT_JOB_NUSHT2D2:
+    case NUSHT_JOB_NUSHT2D1:
+      NUSHT_REQUIRE(!j.ring, kErrArg, at + "nusht2d2 / nusht2d1 jobs take no ring plan");
+      break;
+    default:
+      NUSHT_REQUIRE(false, kErrArg, at + "unknown job kind " + std::to_string(j.kind));
+  }
+  const bool fwd = j.kind == NUSHT_JOB_GCLM2LENMAP || j.kind == NUSHT_JOB_NUSHT2D2;
+  if (fwd) {
+    NUSHT_REQUIRE(j.alm_in && j.map_out0, kErrArg, at + "null alm_in or map_out0");
+    NUSHT_REQUIRE(p->spin == 0 || j.map_out1, kErrArg, at + "spin > 0 needs map_out1");
+  } else {
+    NUSHT_REQUIRE(j.map_in0 && j.alm_out, kErrArg, at + "null map_in0 or alm_out");
+    NUSHT_REQUIRE(p->spin == 0 || j.map_in1, kErrArg, at + "spin > 0 needs map_in1");
+  }
+}
+
+void run_job(const nusht_job& j, hipStream_t st) {
+  switch (j.kind) {
+    case NUSHT_JOB_GCLM2LENMAP:
+      op_gclm2lenmap(j.plan, j.ring, j.alm_in, j.dlm_scaled, j.map_out0, j.map_out1, j.polrot, st);
+      break;
+    case NUSHT_JOB_LENMAP2GCLM:
+      op_lenmap2gclm(j.plan, j.ring, j.map_in0, j.map_in1, j.dlm_scaled, j.alm_out, j.polrot, st);
+      break;
+    case NUSHT_JOB_NUSHT2D2:
+      op_nusht2d2(j.plan, j.alm_in, j.map_out0, j.map_out1, j.gamma, st);
+      break;
+    default:
+      op_nusht2d1(j.plan, j.map_in0, j.map_in1, j.gamma, j.alm_out, st);
+      break;
+  }
+}
+
+int find_root(std::vector<int>& par, int a) {
+  while (par[a] != a) a = par[a] = par[par[a]];
+  return a;
+}
+
+}  // namespace
+
+int nusht_batch_execute(const nusht_job* jobs, int njobs, nusht_stream_t s) {
+  return guard([&] {
+    NUSHT_REQUIRE(njobs >= 0 && (jobs || njobs == 0), kErrArg, "batch: bad job array");
+    for (int i = 0; i < njobs; ++i) check_job(jobs[i], i);
+    if (njobs == 0) return;
+    // jobs that share a plan or a ring plan form one ordered group (union-find over the handles)
+    std::vector<int> par(njobs);
+    for (int i = 0; i < njobs; ++i) par[i] = i;
+    std::unordered_map<const void*, int> owner;
+    for (int i = 0; i < njobs; ++i)
+      for (const void* h : {(const void*)jobs[i].plan, (const void*)jobs[i].ring}) {
+        if (!h) continue;
+        auto it = owner.find(h);
+        if (it == owner.end()) owner.emplace(h, i);
+        else par[find_root(par, i)] = find_root(par, it->second);
+      }
+    std::unordered_map<int, int> group_of_root;
+    std::vector<int> group(njobs);
+    std::vector<double> cost;
+    for (int i = 0; i < njobs; ++i) {
+      const int r = find_root(par, i);
+      auto it = group_of_root.find(r);
+      if (it == group_of_root.end()) it = group_of_root.emplace(r, (int)cost.size()).first, cost.push_back(0.0);
+      group[i] = it->second;
+      const nusht_plan_t p = jobs[i].plan;
+      cost[group[i]] += (double)(p->lmax + 1) * (p->lmax + 1) * (p->spin ? 2.0 : 1.0) * p->np.ntheta;
+    }
+    const int ngroups = (int)cost.size();
+    BatchStreams& bs = batch_streams();
+    std::lock_guard<std::mutex> lock(bs.mu);
+    if (!bs.ready) {
+      for (int k = 0; k < BatchStreams::K; ++k) {
+        NUSHT_HIP(hipStreamCreateWithFlags(&bs.s[k], hipStreamNonBlocking));
+        NUSHT_HIP(hipEventCreateWithFlags(&bs.join[k], hipEventDisableTiming));
+      }
+      NUSHT_HIP(hipEventCreateWithFlags(&bs.fork, hipEventDisableTiming));
+      bs.ready = true;
+    }
+    // longest group first onto the least loaded stream
+    const int K = std::min(BatchStreams::K, ngroups);
+    std::vector<int> order(ngroups), stream_of(ngroups);
+    for (int g = 0; g < ngroups; ++g) order[g] = g;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<double> load(K, 0.0);
+    for (int g : order) {
+      const int k = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      stream_of[g] = k;
+      load[k] += cost[g];
+    }
+    NUSHT_HIP(hipEventRecord(bs.fork, S(s)));
+    for (int k = 0; k < K; ++k) NUSHT_HIP(hipStreamWaitEvent(bs.s[k], bs.fork, 0));
+    for (int i = 0; i < njobs; ++i) run_job(jobs[i], bs.s[stream_of[group[i]]]);
+    for (int k = 0; k < K; ++k) {
+      NUSHT_HIP(hipEventRecord(bs.join[k], bs.s[k]));
+      NUSHT_HIP(hipStreamWaitEvent(S(s), bs.join[k], 0));
+    }
   });
 }
 

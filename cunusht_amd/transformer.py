@@ -111,8 +111,9 @@ class PointPlan:
 class NuPlan:
     """One resident nuSHT plan (lmax, spin, ncomp, epsilon)."""
 
-    def __init__(self, lmax: int, spin: int, ncomp: int, epsilon: float, npts_max: int, sigma: float = 0.0):
-        d = _lib.PlanDesc(lmax, lmax, spin, ncomp, float(epsilon), int(npts_max), float(sigma))
+    def __init__(self, lmax: int, spin: int, ncomp: int, epsilon: float, npts_max: int, sigma: float = 0.0,
+                 max_maps: int = 0):
+        d = _lib.PlanDesc(lmax, lmax, spin, ncomp, float(epsilon), int(npts_max), float(sigma), int(max_maps))
         h = ctypes.c_void_p()
         _lib.call("nusht_plan_create", ctypes.byref(d), _stream(), ctypes.byref(h))
         self._h = h
@@ -123,6 +124,10 @@ class NuPlan:
         i = _lib.PlanInfo()
         _lib.call("nusht_plan_get_info", self._h, ctypes.byref(i))
         return {f: getattr(i, f) for f, _ in _lib.PlanInfo._fields_}
+
+    def nbytes(self) -> int:
+        """Device bytes held by the plan."""
+        return int(_lib.load().nusht_plan_workspace_bytes(self._h))
 
     def set_pts(self, theta: torch.Tensor, phi: torch.Tensor, checked: bool = True):
         """checked: bounds-check the points (one stream synchronization, raises on a bad point);

@@ -112,6 +112,7 @@ typedef struct {
   int legendre_variant;          /* synthesis kernel variant chosen by the plan-time autotuner */
   float legendre_ms;             /* its measured time for one map */
   int ring_lengths;              /* distinct ring lengths (one DFT description each) */
+  int64_t workspace_bytes;       /* device bytes the plan holds */
 } nusht_ringplan_info;
 /* Plan properties (no counterpart in the reference; reported for measurement). */
 int nusht_ringplan_get_info(nusht_ringplan_t p, nusht_ringplan_info* info);
@@ -156,6 +157,7 @@ typedef struct {
   double epsilon;       /* target accuracy; FFT + nuFFT in FP32 when epsilon >= 1e-6 */
   int64_t npts_max;     /* largest point set passed to set_pts */
   double sigma;         /* fine-grid oversampling, 0 = automatic */
+  int max_maps;         /* maps per nusht_nusht2d2_batch call the plan holds buffers for (1..8; 0 = 8) */
 } nusht_plan_desc;
 
 typedef struct {
@@ -177,6 +179,9 @@ typedef struct {
 int nusht_plan_create(const nusht_plan_desc* desc, nusht_stream_t s, nusht_plan_t* out);
 int nusht_plan_destroy(nusht_plan_t p);
 int nusht_plan_get_info(nusht_plan_t p, nusht_plan_info* info);
+/* Device bytes the plan holds (tables and work buffers; the same as workspace_bytes of the info), or -1 for
+ * a null plan.  For HBM-budget admission; no counterpart in the reference, which allocates per call. */
+int64_t nusht_plan_workspace_bytes(nusht_plan_t p);
 
 /* Points (theta in [0, pi], any phi).  Replaces set_pts / cuFINUFFT setpts
  * (GPU_nufft_transformer.py:249-251).  Synchronizes the stream once (bounds check). */
@@ -208,6 +213,37 @@ int nusht_lenmap2gclm(nusht_plan_t p, nusht_ringplan_t ptg, const double* lenmap
  * points, sharing one Legendre recursion per (m, ring); outputs at out0 + k * out_stride (and out1). */
 int nusht_nusht2d2_batch(nusht_plan_t p, int nmaps, const double* alm, int64_t alm_stride, double* out0,
                          double* out1, int64_t out_stride, const double* gamma, nusht_stream_t s);
+
+/* ---------------------------------------------------------------- batches of independent transforms
+ * No counterpart in the reference (one call at a time on the legacy stream).  A batch runs jobs of
+ * any lmax, spin and epsilon, each on its own plan(s): jobs that share a plan or ring plan run in
+ * submission order on one internal stream, the other groups run concurrently on up to four internal
+ * streams (default priority), forked from and joined back into `s`.  The call only enqueues work, so a
+ * caller may capture it into a hipGraph on `s` and replay it.  All jobs are validated before any work is
+ * enqueued.  Points of NUSHT_JOB_NUSHT2D2 / NUSHT_JOB_NUSHT2D1 jobs are the plan's current points. */
+typedef enum {
+  NUSHT_JOB_GCLM2LENMAP = 0,  /* gclm2lenmap: alm_in, dlm_scaled -> map_out0 (, map_out1) */
+  NUSHT_JOB_LENMAP2GCLM = 1,  /* lenmap2gclm: map_in0 (, map_in1), dlm_scaled -> alm_out */
+  NUSHT_JOB_NUSHT2D2 = 2,     /* nusht2d2: alm_in -> map_out0 (, map_out1) at the plan's points */
+  NUSHT_JOB_NUSHT2D1 = 3      /* nusht2d1: map_in0 (, map_in1) at the plan's points -> alm_out */
+} nusht_job_kind;
+
+typedef struct {
+  int kind;                   /* nusht_job_kind */
+  nusht_plan_t plan;
+  nusht_ringplan_t ring;      /* lensing jobs: spin-1 ring plan of the pointing grid (else NULL) */
+  const double* dlm_scaled;   /* lensing jobs */
+  int polrot;                 /* lensing jobs: rotate spin-s fields by exp(i s gamma) */
+  const double* gamma;        /* nusht2d2 / nusht2d1 jobs: optional per-point rotation angle */
+  const double* alm_in;
+  double* alm_out;
+  const double* map_in0;
+  const double* map_in1;
+  double* map_out0;
+  double* map_out1;
+} nusht_job;
+
+int nusht_batch_execute(const nusht_job* jobs, int njobs, nusht_stream_t s);
 
 /* ---------------------------------------------------------------- stage timing (no counterpart in the
  * reference, whose timer decorators synchronize the device around every stage: helper_GPU.py:10-51).
