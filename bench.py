@@ -173,6 +173,66 @@ def serve_headline(args, L, eps, gclms, dlms, rng):
             "plan_bytes": st["plan_bytes"], "wall_s": st["wall_s"]}
 
 
+def uneven_batch(njobs: int, steps: int, warmup: int):
+    """Independent transforms of uneven size: lmax ~ U{511, 1023, 1535, 2047}, spin ~ U{0, 2}, eps 1e-7, each
+    with its own seeded gclm (C_l ~ (l+1)^-2) and deflection (2.5 arcmin rms, C_l ~ l^-2).  Jobs of one
+    (lmax, spin) share two resident plans.  Timed: the jobs as sequential single calls on one stream, the same
+    jobs through nusht_batch_execute, and the batch captured once as a hipGraph and replayed."""
+    import torch
+    from cunusht_amd.batch import BatchJob, CapturedBatch, LensingPlan, execute
+    from cunusht_amd.utils import synalm
+    rng = np.random.default_rng(44)
+    lmaxs = rng.choice([511, 1023, 1535, 2047], njobs)
+    spins = rng.choice([0, 2], njobs)
+    plans, jobs = {}, []
+    for i, (L, s) in enumerate(zip(lmaxs, spins)):
+        L, s = int(L), int(s)
+        pl = plans.setdefault((L, s), [LensingPlan(L, s, 1e-7), LensingPlan(L, s, 1e-7)])
+        ll = np.arange(L + 1, dtype=np.float64)
+        cl = 1.0 / (ll + 1.0) ** 2
+        nc = 1 if s == 0 else 2
+        g = np.stack([synalm(cl, L, rng=rng) for _ in range(nc)])
+        cldd = np.zeros(L + 1)
+        cldd[2:] = ll[2:] ** -2.0
+        cldd *= (2.5 / 60.0 * math.pi / 180.0) ** 2 / (np.sum((2 * ll + 1) * cldd) / (4 * math.pi))
+        with np.errstate(divide="ignore"):
+            fl = np.where(ll > 0, 1.0 / np.sqrt(ll * (ll + 1)), 0.0)
+        from cunusht_amd.utils import almxfl
+        d = almxfl(synalm(cldd, L, rng=rng), fl, L, False)
+        jobs.append(BatchJob("gclm2lenmap", pl[i % 2], torch.as_tensor(d, device="cuda"),
+                             torch.as_tensor(g, device="cuda")))
+    from cunusht_amd import _lib
+    import ctypes
+    sp = lambda: ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+    def sequential():
+        for j in jobs:
+            cj = j.c_job()
+            _lib.call("nusht_gclm2lenmap", ctypes.c_void_p(cj.plan), ctypes.c_void_p(cj.ring), ctypes.c_void_p(cj.alm_in),
+                      ctypes.c_void_p(cj.dlm_scaled), ctypes.c_void_p(cj.map_out0),
+                      ctypes.c_void_p(cj.map_out1) if cj.map_out1 else None, cj.polrot, sp())
+
+    def timed(fn):
+        for _ in range(warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps
+
+    t_seq = timed(sequential)
+    t_batch = timed(lambda: execute(jobs))
+    cap = CapturedBatch(jobs)
+    t_graph = timed(cap.replay)
+    return {"metric": f"{njobs} uneven lensing transforms per pass (lmax 511..2047, spin 0/2, eps 1e-7)",
+            "unit": "ms per pass", "sequential_ms": t_seq * 1e3, "batch_ms": t_batch * 1e3, "graph_ms": t_graph * 1e3,
+            "graph_speedup": t_seq / t_graph, "jobs_per_s_graph": njobs / t_graph,
+            "plans": sum(len(v) for v in plans.values()), "steps": steps, "warmup": warmup,
+            "mix": {f"{L}/{s}": int(np.sum((lmaxs == L) & (spins == s))) for (L, s) in sorted(plans)}}
+
+
 def geometry_ntheta_ref(lmax: int) -> int:
     from oracle.geometry import reference_cc_sizes
     return reference_cc_sizes(lmax)[0]
@@ -191,6 +251,9 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=32,
                     help="requests replayed through the serving loop (0 = skip); they share 4 deflections")
     ap.add_argument("--serve-rate", type=float, default=400.0, help="Poisson arrival rate of served requests (1/s)")
+    ap.add_argument("--uneven-batch", type=int, default=0,
+                    help="instead of the headline: N independent gclm2lenmap jobs of lmax in {511, 1023, 1535, 2047}, "
+                         "spin in {0, 2}, eps 1e-7, as one hipGraph-captured batch vs N sequential calls")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the timed region, copy a buffer of this many MiB once (a plain copy kernel "
                          "of known traffic, used to calibrate FETCH_SIZE / WRITE_SIZE counters)")
@@ -205,6 +268,9 @@ def main():
     from cunusht_amd import _lib
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.uneven_batch > 0:
+        print(json.dumps(uneven_batch(args.uneven_batch, args.steps, args.warmup)))
+        return
     L, eps, B = args.lmax, args.eps, args.batch
     print(f"[bench] lmax={L} eps={eps:g} batch={B} steps={args.steps} warmup={args.warmup}", file=sys.stderr,
           flush=True)

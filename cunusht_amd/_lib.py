@@ -88,6 +88,14 @@ SIGNATURES = {
     "nusht_plan_destroy": [_vp],
     "nusht_plan_get_info": [_vp, ctypes.POINTER(PlanInfo)],
     "nusht_plan_workspace_bytes": [_vp],
+    "nusht_sht_synth_grad_gl": [_vp, _vp, _vp, _vp, _vp],
+    "nusht_fft2d_c2c_f64": [_vp, _i, _i, _i, _i, _vp],
+    "nusht_fft2d_c2c_f32": [_vp, _i, _i, _i, _i, _vp],
+    "nusht_sht_synth_cc": [_vp, _vp, _vp, _vp],
+    "nusht_sht_adjoint_cc": [_vp, _vp, _vp, _vp],
+    "nusht_nufft_setpts": [_vp, _vp, _vp, _i64, _vp],
+    "nusht_nufft_interp": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "nusht_nufft_spread": [_vp, _vp, _vp, _vp, _vp, _vp],
     "nusht_batch_execute": [ctypes.POINTER(Job), _i, _vp],
     "nusht_plan_set_pts": [_vp, _vp, _vp, _i64, _vp],
     "nusht_plan_set_pts_async": [_vp, _vp, _vp, _i64, _vp],

@@ -9,6 +9,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "fft2d.h"
 #include "legendre.h"
 #include "nufft.h"
 #include "points.h"
@@ -218,6 +219,14 @@ int nusht_ringplan_adjoint(nusht_ringplan_t p, const double* map0, const double*
   });
 }
 
+int nusht_sht_synth_grad_gl(nusht_ringplan_t p, const double* alm, double* dtheta, double* dphi, nusht_stream_t s) {
+  return guard([&] {
+    NUSHT_REQUIRE(p, kErrArg, "null ring plan");
+    NUSHT_REQUIRE(alm && dtheta && dphi, kErrArg, "null array");
+    p->rp->synthesis_grad(reinterpret_cast<const double2*>(alm), dtheta, dphi, S(s));
+  });
+}
+
 int nusht_ringplan_dlm2pointing(nusht_ringplan_t p, const double* dlm_scaled, double* ptg_theta, double* ptg_phi,
                                 double* gamma, nusht_stream_t s) {
   return guard([&] {
@@ -403,6 +412,64 @@ int nusht_lenmap2gclm(nusht_plan_t p, nusht_ringplan_t ptg, const double* lenmap
     NUSHT_REQUIRE(lenmap0 && dlm_scaled && gclm, kErrArg, "null array");
     NUSHT_REQUIRE(p->spin == 0 || lenmap1, kErrArg, "spin > 0 needs two input arrays");
     op_lenmap2gclm(p, ptg, lenmap0, lenmap1, dlm_scaled, gclm, polrot, S(s));
+  });
+}
+
+// ------------------------------------------------------------------ 2D FFT
+int nusht_fft2d_c2c_f64(double* data, int n0, int n1, int batch, int direction, nusht_stream_t s) {
+  return guard([&] {
+    NUSHT_REQUIRE(direction == -1 || direction == 1, kErrArg, "direction must be -1 or +1");
+    fft2d_c2c(reinterpret_cast<double2*>(data), n0, n1, batch, direction > 0, S(s));
+  });
+}
+
+int nusht_fft2d_c2c_f32(float* data, int n0, int n1, int batch, int direction, nusht_stream_t s) {
+  return guard([&] {
+    NUSHT_REQUIRE(direction == -1 || direction == 1, kErrArg, "direction must be -1 or +1");
+    fft2d_c2c(reinterpret_cast<float2*>(data), n0, n1, batch, direction > 0, S(s));
+  });
+}
+
+// ------------------------------------------------------------------ plan stages
+int nusht_sht_synth_cc(nusht_plan_t p, const double* alm, double* G, nusht_stream_t s) {
+  return guard([&] {
+    check_plan(p);
+    NUSHT_REQUIRE(alm && G, kErrArg, "null array");
+    legendre_synth(p->tab, 1, p->ncomp, reinterpret_cast<const double2*>(alm), 0, nullptr, p->coef.as<double2>(),
+                   reinterpret_cast<double2*>(G), p->np.ntheta, 1, 0, S(s));
+  });
+}
+
+int nusht_sht_adjoint_cc(nusht_plan_t p, const double* G, double* alm, nusht_stream_t s) {
+  return guard([&] {
+    check_plan(p);
+    NUSHT_REQUIRE(alm && G, kErrArg, "null array");
+    legendre_adjoint(p->tab, p->ncomp, reinterpret_cast<const double2*>(G), p->np.ntheta, 1,
+                     reinterpret_cast<double2*>(alm), S(s));
+  });
+}
+
+int nusht_nufft_setpts(nusht_plan_t p, const double* theta, const double* phi, int64_t npts, nusht_stream_t s) {
+  return nusht_plan_set_pts(p, theta, phi, npts, s);
+}
+
+int nusht_nufft_interp(nusht_plan_t p, const double* G, const double* gamma, double* out0, double* out1,
+                       nusht_stream_t s) {
+  return guard([&] {
+    check_plan(p);
+    NUSHT_REQUIRE(G && out0, kErrArg, "null array");
+    NUSHT_REQUIRE(p->spin == 0 || out1, kErrArg, "spin > 0 needs two output arrays");
+    p->nu->type2(reinterpret_cast<const double2*>(G), p->np.ntheta, gamma, out0, out1, S(s));
+  });
+}
+
+int nusht_nufft_spread(nusht_plan_t p, const double* in0, const double* in1, const double* gamma, double* G,
+                       nusht_stream_t s) {
+  return guard([&] {
+    check_plan(p);
+    NUSHT_REQUIRE(in0 && G, kErrArg, "null array");
+    NUSHT_REQUIRE(p->spin == 0 || in1, kErrArg, "spin > 0 needs two input arrays");
+    p->nu->type1(in0, in1, gamma, reinterpret_cast<double2*>(G), p->np.ntheta, S(s));
   });
 }
 

@@ -61,6 +61,8 @@ class RingPlan {
   // for band-limited maps on a Gauss-Legendre grid)
   void analysis(const double* map0, const double* map1, double2* alm, int ncomp, hipStream_t st);
   bool has_weights() const { return weight_.bytes() != 0; }
+  // spin-1 plans: gradient (d/dtheta, d/dphi / sin theta) of the scalar field with coefficients alm
+  void synthesis_grad(const double2* alm, double* dtheta, double* dphi, hipStream_t st);
   // spin-1 GRAD synthesis of dlm = sqrt(l(l+1)) dlm_scaled, then deflected angles (and gamma)
   void dlm2pointing(const double2* dlm_scaled, double* ptheta, double* pphi, double* gamma, hipStream_t st);
 

@@ -512,6 +512,12 @@ void RingPlan::analysis(const double* map0, const double* map1, double2* alm, in
   legendre_adjoint(tab_, ncomp, G_.as<double2>(), gstride_, 1, alm, st);
 }
 
+void RingPlan::synthesis_grad(const double2* alm, double* dtheta, double* dphi, hipStream_t st) {
+  NUSHT_REQUIRE(spin_ == 1, kErrState, "gradient synthesis needs a spin-1 ring plan");
+  legendre_synth(tab_, 1, 1, alm, 0, sqll_.as<double>(), coef_.as<double2>(), G_.as<double2>(), gstride_, 1, 0, st);
+  ring_pass(1, dtheta, dphi, nullptr, st);
+}
+
 void RingPlan::dlm2pointing(const double2* dlm_scaled, double* ptheta, double* pphi, double* gamma, hipStream_t st) {
   NUSHT_REQUIRE(spin_ == 1, kErrState, "dlm2pointing needs a spin-1 ring plan");
   legendre_synth(tab_, 1, 1, dlm_scaled, 0, sqll_.as<double>(), coef_.as<double2>(), G_.as<double2>(), gstride_, 1, 0,

@@ -130,6 +130,9 @@ int nusht_ringplan_analysis(nusht_ringplan_t p, const double* map0, const double
 /* exact adjoint of the synthesis (no weights) */
 int nusht_ringplan_adjoint(nusht_ringplan_t p, const double* map0, const double* map1, double* alm, int ncomp,
                            nusht_stream_t s);
+/* spin-1 plans: gradient of a scalar field, (d/dtheta, d/dphi / sin theta) of f = sum alm Y_lm on the ring
+ * grid.  Replaces synthesis_der1_cupy / SHTns synth_grad (cunusht/sht/GPU_sht_transformer.py:88-96). */
+int nusht_sht_synth_grad_gl(nusht_ringplan_t p, const double* alm, double* dtheta, double* dphi, nusht_stream_t s);
 /* spin-1 plans: dlm_scaled = dlm / sqrt(l(l+1)) -> deflected (theta, phi) and gamma (may be NULL) */
 int nusht_ringplan_dlm2pointing(nusht_ringplan_t p, const double* dlm_scaled, double* ptg_theta, double* ptg_phi,
                                 double* gamma, nusht_stream_t s);
@@ -208,6 +211,35 @@ int nusht_gclm2lenmap(nusht_plan_t p, nusht_ringplan_t ptg, const double* gclm, 
                       double* lenmap0, double* lenmap1, int polrot, nusht_stream_t s);
 int nusht_lenmap2gclm(nusht_plan_t p, nusht_ringplan_t ptg, const double* lenmap0, const double* lenmap1,
                       const double* dlm_scaled, double* gclm, int polrot, nusht_stream_t s);
+
+/* Batched in-place 2D complex FFT of `batch` row-major n0 x n1 arrays (interleaved complex), forward
+ * (direction -1, sign -1) or backward (direction +1, sign +1, unnormalised) as cuFFT C2C.  Replaces the
+ * reference's _C2C / _iC2C (GPU_nufft_transformer.py:271-287).  n0 and n1 must factor into 2, 3, 5, 7, 13
+ * and not exceed 10240 (f64) / 20480 (f32).  The first call for a length makes its twiddle table (one
+ * device allocation kept for the life of the process); later calls allocate nothing. */
+int nusht_fft2d_c2c_f64(double* data, int n0, int n1, int batch, int direction, nusht_stream_t s);
+int nusht_fft2d_c2c_f32(float* data, int n0, int n1, int batch, int direction, nusht_stream_t s);
+
+/* Stages of the plan, for callers that compose them (the reference's _synthesis / _nuFFT2d2 / _nuFFT2d1 /
+ * _adjoint_synthesis, GPU_nufft_transformer.py:254-310).  The intermediate G holds the Fourier coefficients
+ * in phi of the synthesis on the plan's equiangular rings theta_j = pi j / (ntheta - 1), j = 0..ntheta-1
+ * (ntheta from nusht_plan_info; the rings include both poles): G[r * ntheta + j] complex, rows r = m for
+ * spin 0 (m = 0..lmax) and r = m + lmax for spin s (m = -lmax..lmax), so that the field on ring j is
+ * f(phi) = sum_m G_m(theta_j) e^{i m phi} (real part 2 Re sum_{m>0} + G_0 for spin 0; Q + iU for spin s).
+ *   nusht_sht_synth_cc:   alm -> G                  (SHT synthesis on the rings)
+ *   nusht_sht_adjoint_cc: G -> alm                  (its exact adjoint)
+ *   nusht_nufft_interp:   G -> values at the plan's points (doubling onto the torus, 2D FFT, ES
+ *                         interpolation; rotation by exp(i s gamma) if gamma != NULL)
+ *   nusht_nufft_spread:   values -> G               (exact adjoint of nusht_nufft_interp)
+ * so nusht_nusht2d2 = synth_cc + nufft_interp and nusht_nusht2d1 = nufft_spread + adjoint_cc.
+ * nusht_nufft_setpts is nusht_plan_set_pts (cuFINUFFT setpts, GPU_nufft_transformer.py:249-251). */
+int nusht_sht_synth_cc(nusht_plan_t p, const double* alm, double* G, nusht_stream_t s);
+int nusht_sht_adjoint_cc(nusht_plan_t p, const double* G, double* alm, nusht_stream_t s);
+int nusht_nufft_setpts(nusht_plan_t p, const double* theta, const double* phi, int64_t npts, nusht_stream_t s);
+int nusht_nufft_interp(nusht_plan_t p, const double* G, const double* gamma, double* out0, double* out1,
+                       nusht_stream_t s);
+int nusht_nufft_spread(nusht_plan_t p, const double* in0, const double* in1, const double* gamma, double* G,
+                       nusht_stream_t s);
 
 /* Batched synthesis: nmaps alm sets (stride alm_stride doubles between sets) on the plan's current
  * points, sharing one Legendre recursion per (m, ring); outputs at out0 + k * out_stride (and out1). */
