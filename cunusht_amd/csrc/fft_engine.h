@@ -278,6 +278,7 @@ class FftLength {
   void init(int n);
   bool valid() const { return n_ > 0; }
   int n() const { return n_; }
+  size_t bytes() const { return tw_.bytes(); }
   FftDesc<T> desc() const {
     FftDesc<T> d{};
     d.n = n_;

@@ -13,30 +13,13 @@
 #pragma once
 
 #include "common.h"
-#include "fft_engine.h"
+#include "dft_any.h"
 #include "legendre.h"
-
-#include <map>
-#include <memory>
 
 namespace nusht {
 
-enum class RingMethod : int { Direct = 0, Rader = 1, Bluestein = 2, Split = 3 };
-constexpr int kRingMethods = 4;
-
-// One distinct ring length and how its DFT runs in LDS (kernel-argument view).
-struct RingLen {
-  int n;              // ring length
-  int method;         // RingMethod
-  int mlen;           // FFT length run in LDS: n (direct), n - 1 (Rader), padded (Bluestein, split sub)
-  int r, b;           // split: n = r * b, r sub-transforms of length b
-  int sub_bluestein;  // split: sub-transforms by Bluestein (else a direct FFT of length b)
-  FftDesc<double> fd; // of length mlen
-  const double2* chirp;  // Bluestein: e^{-i pi j^2 / len}, len = n (or b for split)
-  const double2* kern;   // Bluestein: FFT of the wrapped conjugate chirp (mlen); Rader: FFT of the permuted roots
-  const int* perm_in;    // Rader: g^q mod n
-  const int* perm_out;   // Rader: g^-q mod n
-};
+using RingMethod = DftMethod;
+using RingLen = DftLen<double>;
 
 class RingPlan {
  public:
@@ -49,7 +32,7 @@ class RingPlan {
   int spin() const { return spin_; }
   long npix() const { return npix_; }
   int nring() const { return nring_; }
-  int nlengths() const { return (int)lens_h_.size(); }
+  int nlengths() const { return dft_.size(); }
   RingMethod method() const { return method_; }  // method of the longest ring
   const LegendreTables& tables() const { return tab_; }
 
@@ -69,27 +52,19 @@ class RingPlan {
   size_t bytes() const;
 
  private:
-  struct LenTables {
-    RingLen d{};
-    DevBuf chirp, kern, perm_in, perm_out;
-  };
-  int add_length(int n, hipStream_t st);
-  const FftLength<double>& fft_of(int len);
   void ring_pass(int mode, double* o0, double* o1, double* o2, hipStream_t st, const double* wt = nullptr);
 
   int lmax_, spin_, nring_;
   long npix_;
   RingMethod method_ = RingMethod::Direct;
   LegendreTables tab_;
-  std::map<int, std::unique_ptr<FftLength<double>>> ffts_;  // by FFT length
-  std::vector<std::unique_ptr<LenTables>> lens_h_;
-  std::map<int, int> len_index_;                             // ring length -> index into lens_h_
+  DftSet<double> dft_;                                       // one DFT description per distinct ring length
   DevBuf lens_;                                              // RingLen[nlengths]
   DevBuf lenidx_;                                            // int[nring]
-  DevBuf rings_[kRingMethods];                               // ring ids per DFT method
-  int nrings_m_[kRingMethods] = {0, 0, 0, 0};
-  int uniform_[kRingMethods] = {-1, -1, -1, -1};             // the one length index of a method's rings, or -2
-  size_t lds_[kRingMethods] = {0, 0, 0, 0};
+  DevBuf rings_[kDftMethods];                                // ring ids per DFT method
+  int nrings_m_[kDftMethods] = {0, 0, 0, 0};
+  int uniform_[kDftMethods] = {-1, -1, -1, -1};              // the one length index of a method's rings, or -2
+  size_t lds_[kDftMethods] = {0, 0, 0, 0};
   DevBuf theta_, phi0_, ofs_, weight_;
   DevBuf sqll_;  // sqrt(l(l+1))
   DevBuf G_;     // [nrows][nring] complex (m-major)

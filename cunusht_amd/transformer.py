@@ -23,7 +23,7 @@ import warnings
 import numpy as np
 import torch
 
-from . import _lib
+from . import _lib, diagnostics
 from .geometry import Geom, get_geom
 from .utils import Alm
 
@@ -164,6 +164,7 @@ class GPU_nusht_transformer:
         self.dlm_scaled = None
         self.ptg = None
         self.ret = {}
+        self.timer = diagnostics.Timer()
         self._plans = {}
         self._rings = {}
         self._ptg_key = None
@@ -368,8 +369,24 @@ class GPU_nusht_transformer:
     def gclm2lenmap(self, gclm, lmax, mmax, ptg=None, dlm_scaled=None, epsilon=None, lenmap=None, verbose=1,
                     execmode="normal", runid=None, GPUdesc="", spin=None, polrot=True):
         """Lensed map(s) at the deflected pointing grid (GPU_nufft_transformer.py:486-578;
-        spin-s with rotation as CPU_nufft_transformer.py:429-484)."""
+        spin-s with rotation as CPU_nufft_transformer.py:429-484).  execmode 'timing' stores and dumps the
+        stage times (self.timer.keys), 'debug' returns the reference's intermediate arrays (self.ret)."""
         assert execmode in ["normal", "debug", "timing"]
+        self.execmode = execmode
+        run = lambda: self._gclm2lenmap(gclm, lmax, mmax, ptg, dlm_scaled, epsilon, lenmap, spin, polrot)
+        if execmode == "timing":
+            return diagnostics.timed(self, "gclm2lenmap", lmax, runid, GPUdesc, run)
+        res = run()
+        if execmode == "debug":
+            sp = self.spin if spin is None else spin
+            g = _dev(gclm, torch.complex128, "gclm")
+            self.ret = diagnostics.debug_forward(self, g, sp, self._ncomp(g, sp), res)
+            self.ret["dlm2pointing"] = self.ptg.cpu().numpy()
+            print("::debug:: Returned component results")
+            return self.ret
+        return res
+
+    def _gclm2lenmap(self, gclm, lmax, mmax, ptg, dlm_scaled, epsilon, lenmap, spin, polrot):
         self._check_lm(lmax, mmax)
         spin = self.spin if spin is None else spin
         eps = self._eps(epsilon)
@@ -389,8 +406,22 @@ class GPU_nusht_transformer:
 
     def lenmap2gclm(self, lenmap, dlm_scaled, gclm, lmax, mmax, epsilon=None, ptg=None, verbose=1,
                     execmode="normal", runid=None, GPUdesc="", spin=None, polrot=True):
-        """Exact adjoint of gclm2lenmap (GPU_nufft_transformer.py:580-673)."""
+        """Exact adjoint of gclm2lenmap (GPU_nufft_transformer.py:580-673).  execmode 'timing' as for
+        gclm2lenmap; 'debug' returns self.ret with the pointing and the result ('adjointsynthesis'): the
+        adjoint chain is fused and forms no reference-layout intermediate."""
         assert execmode in ["normal", "debug", "timing"]
+        self.execmode = execmode
+        run = lambda: self._lenmap2gclm(lenmap, dlm_scaled, gclm, lmax, mmax, epsilon, ptg, spin, polrot)
+        if execmode == "timing":
+            return diagnostics.timed(self, "lenmap2gclm", lmax, runid, GPUdesc, run)
+        res = run()
+        if execmode == "debug":
+            self.ret = {"dlm2pointing": self.ptg.cpu().numpy(), "adjointsynthesis": res.cpu().numpy()}
+            print("::debug:: Returned component results")
+            return self.ret
+        return res
+
+    def _lenmap2gclm(self, lenmap, dlm_scaled, gclm, lmax, mmax, epsilon, ptg, spin, polrot):
         self._check_lm(lmax, mmax)
         spin = self.spin if spin is None else spin
         eps = self._eps(epsilon)

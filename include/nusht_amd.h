@@ -214,9 +214,10 @@ int nusht_lenmap2gclm(nusht_plan_t p, nusht_ringplan_t ptg, const double* lenmap
 
 /* Batched in-place 2D complex FFT of `batch` row-major n0 x n1 arrays (interleaved complex), forward
  * (direction -1, sign -1) or backward (direction +1, sign +1, unnormalised) as cuFFT C2C.  Replaces the
- * reference's _C2C / _iC2C (GPU_nufft_transformer.py:271-287).  n0 and n1 must factor into 2, 3, 5, 7, 13
- * and not exceed 10240 (f64) / 20480 (f32).  The first call for a length makes its twiddle table (one
- * device allocation kept for the life of the process); later calls allocate nothing. */
+ * reference's _C2C / _iC2C (GPU_nufft_transformer.py:271-287).  Sides that factor into 2, 3, 5, 7, 13 may
+ * reach 10240 (f64) / 20480 (f32); any other side (Rader, Bluestein or split transforms) up to 8320, except a
+ * prime above 4160 whose n - 1 is not smooth.  The first call for a length makes its tables (device
+ * allocations kept for the life of the process, one stream synchronization); later calls allocate nothing. */
 int nusht_fft2d_c2c_f64(double* data, int n0, int n1, int batch, int direction, nusht_stream_t s);
 int nusht_fft2d_c2c_f32(float* data, int n0, int n1, int batch, int direction, nusht_stream_t s);
 

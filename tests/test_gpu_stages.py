@@ -50,10 +50,28 @@ def test_fft2d_vs_numpy(shape, prec, direction):
 
 
 def test_fft2d_rejects_unsupported_length():
+    """4177 is prime with 4176 = 2^4 3^2 29 and too long for Bluestein in LDS; 8501 exceeds the any-length
+    limit; both are refused on the host before any launch."""
     from cunusht_amd import _lib
-    d = torch.zeros((11, 8), dtype=torch.complex128, device="cuda")
-    with pytest.raises(_lib.NuShtError, match="factor"):
-        _lib.call("nusht_fft2d_c2c_f64", _p(d), 11, 8, 1, -1, _s())
+    d = torch.zeros((2, 8501), dtype=torch.complex128, device="cuda")
+    with pytest.raises(_lib.NuShtError, match="not supported"):
+        _lib.call("nusht_fft2d_c2c_f64", _p(d), 1, 4177, 1, -1, _s())
+    with pytest.raises(_lib.NuShtError, match="exceeds"):
+        _lib.call("nusht_fft2d_c2c_f64", _p(d), 2, 8501, 1, -1, _s())
+
+
+@pytest.mark.parametrize("shape", [(11, 17), (22, 89), (97, 44), (8191, 2), (3, 4124)])
+def test_fft2d_any_length_vs_numpy(shape):
+    """Sides by Rader (11, 17, 97, 8191), Bluestein (22, 44, 89) and a split transform (4124 = 2 x 2062)."""
+    from cunusht_amd import _lib
+    rng = np.random.default_rng(shape[0] + 3 * shape[1])
+    x = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    d = _t(x)
+    _lib.call("nusht_fft2d_c2c_f64", _p(d), shape[0], shape[1], 1, -1, _s())
+    assert _relmax(d.cpu().numpy(), np.fft.fft2(x)) < 1e-12
+    f = torch.as_tensor(x, device="cuda").to(torch.complex64).contiguous()
+    _lib.call("nusht_fft2d_c2c_f32", _p(f), shape[0], shape[1], 1, 1, _s())
+    assert _relmax(f.cpu().numpy(), np.fft.ifft2(x) * x.size) < 2e-5
 
 
 def _plan(L, spin, eps, npts):
