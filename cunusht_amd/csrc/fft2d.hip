@@ -37,7 +37,7 @@ __global__ __launch_bounds__(kFft2dNT) void k_fft_rows(c2_t<T>* data, int n0, in
     buf[k] = backward ? cconj(v) : v;
   }
   __syncthreads();
-  fft_lds<T, kFft2dNT, Fft2dCap<T>::n, true>(buf, fd);
+  fft_lds<T, kFft2dNT, Fft2dCap<T>::n, true, true>(buf, fd);
   for (int k = threadIdx.x; k < n1; k += kFft2dNT) {
     const T2 v = buf[k];
     row[k] = backward ? cconj(v) : v;
@@ -59,7 +59,7 @@ __global__ __launch_bounds__(kFft2dNT) void k_fft_cols(c2_t<T>* data, int n0, in
     buf[c * n0 + r] = backward ? cconj(v) : v;
   }
   __syncthreads();
-  for (int c = 0; c < nc; ++c) fft_lds<T, kFft2dNT, Fft2dCap<T>::n, true>(buf + c * n0, fd);
+  for (int c = 0; c < nc; ++c) fft_lds<T, kFft2dNT, Fft2dCap<T>::n, true, true>(buf + c * n0, fd);
   for (int i = threadIdx.x; i < n0 * nc; i += kFft2dNT) {
     const int r = i / nc, c = i - r * nc;
     const T2 v = buf[c * n0 + r];

@@ -201,7 +201,21 @@ template <typename T> struct Dft<T, 13> : DftGeneric<T, 13> {};
 
 // ---------------------------------------------------------------- one stage
 // NT threads; every thread handles up to NB butterflies held in registers.
-template <typename T, int R, int NT, int NMAX>
+// LDS bank spread for the stages in between: within each aligned block of 16 values, value i sits at
+// i ^ ((i >> 4) & 7).  A 16-byte value spans 4 of the 64 banks, so 16 lanes are conflict-free when their
+// values differ in the low 4 index bits; a stage's strided writes (stride R values when ns = 1, runs of ns
+// values R ns apart later) otherwise pile onto a few bank groups (8-way for R = 8).  Among the XOR maps of
+// bits 4..7 onto bits 0..3 this one gives the fewest conflicts over the stage patterns of the radix-8/4/5/3/2
+// lengths used here (mean 1.13 lanes per bank group and 16-lane pass, worst 2; 2.0 and 8 without).  The
+// first stage reads and the last stage writes the natural order, which are contiguous.  Used when n is a
+// multiple of 16, so the permutation stays inside [0, n).
+template <bool ON>
+__device__ __forceinline__ int fft_swz(int i) {
+  if constexpr (ON) return i ^ ((i >> 4) & 7);
+  return i;
+}
+
+template <typename T, int R, int NT, int NMAX, bool SWZ_IN = false, bool SWZ_OUT = false>
 __device__ __forceinline__ void fft_stage(c2_t<T>* buf, int n, int ns, const c2_t<T>* __restrict__ tw) {
   using T2 = c2_t<T>;
   constexpr int NB = (NMAX + R * NT - 1) / (R * NT);
@@ -215,7 +229,7 @@ __device__ __forceinline__ void fft_stage(c2_t<T>* buf, int n, int ns, const c2_
     if (j < nb) {
       int k = j % ns;
 #pragma unroll
-      for (int r = 0; r < R; ++r) v[i][r] = buf[j + r * stride];
+      for (int r = 0; r < R; ++r) v[i][r] = buf[fft_swz<SWZ_IN>(j + r * stride)];
       if (ns > 1) {
         // one table read per butterfly; w^r by repeated multiplication (error ~ R ulp)
         const T2 w1 = tw[k * twstep];
@@ -237,7 +251,7 @@ __device__ __forceinline__ void fft_stage(c2_t<T>* buf, int n, int ns, const c2_
       int k = j % ns;
       int base = (j - k) * R + k;
 #pragma unroll
-      for (int r = 0; r < R; ++r) buf[base + r * ns] = v[i][r];
+      for (int r = 0; r < R; ++r) buf[fft_swz<SWZ_OUT>(base + r * ns)] = v[i][r];
     }
   }
   __syncthreads();
@@ -246,30 +260,47 @@ __device__ __forceinline__ void fft_stage(c2_t<T>* buf, int n, int ns, const c2_
 // Forward FFT of buf[0..n) in LDS.  Caller must __syncthreads() after filling buf.
 // GENERIC enables the odd radices 7 and 13 (needed only for externally imposed ring lengths);
 // lengths chosen internally are 2-3-5 smooth and use radices 8, 4, 2, 3, 5.
-template <typename T, int NT, int NMAX, bool GENERIC = false>
-__device__ void fft_lds(c2_t<T>* buf, const FftDesc<T>& d) {
-  int ns = 1;
-  for (int s = 0; s < d.nstages; ++s) {
-    const int R = d.radix[s];
-    switch (R) {
-      case 2: fft_stage<T, 2, NT, NMAX>(buf, d.n, ns, d.tw); break;
-      case 3: fft_stage<T, 3, NT, NMAX>(buf, d.n, ns, d.tw); break;
-      case 4: fft_stage<T, 4, NT, NMAX>(buf, d.n, ns, d.tw); break;
-      case 5: fft_stage<T, 5, NT, NMAX>(buf, d.n, ns, d.tw); break;
-      case 8: fft_stage<T, 8, NT, NMAX>(buf, d.n, ns, d.tw); break;
-      default:
-        if constexpr (GENERIC) {
-          if (R == 7) fft_stage<T, 7, NT, NMAX>(buf, d.n, ns, d.tw);
-          else if (R == 13) fft_stage<T, 13, NT, NMAX>(buf, d.n, ns, d.tw);
-        }
-        break;
-    }
-    ns *= R;
+template <typename T, int NT, int NMAX, bool GENERIC, bool SI, bool SO>
+__device__ __forceinline__ void fft_stage_r(c2_t<T>* buf, const FftDesc<T>& d, int R, int ns) {
+  switch (R) {
+    case 2: fft_stage<T, 2, NT, NMAX, SI, SO>(buf, d.n, ns, d.tw); break;
+    case 3: fft_stage<T, 3, NT, NMAX, SI, SO>(buf, d.n, ns, d.tw); break;
+    case 4: fft_stage<T, 4, NT, NMAX, SI, SO>(buf, d.n, ns, d.tw); break;
+    case 5: fft_stage<T, 5, NT, NMAX, SI, SO>(buf, d.n, ns, d.tw); break;
+    case 8: fft_stage<T, 8, NT, NMAX, SI, SO>(buf, d.n, ns, d.tw); break;
+    default:
+      if constexpr (GENERIC) {
+        if (R == 7) fft_stage<T, 7, NT, NMAX, SI, SO>(buf, d.n, ns, d.tw);
+        else if (R == 13) fft_stage<T, 13, NT, NMAX, SI, SO>(buf, d.n, ns, d.tw);
+      }
+      break;
   }
 }
 
-// ---------------------------------------------------------------- host-side planning
-// Owns the twiddle table of one length on the device.
+// SWZ: lengths that are multiples of 16 keep the stages in between in the bank-spread order (fft_swz);
+// it costs registers, so kernels that are register-bound leave it off.
+template <typename T, int NT, int NMAX, bool GENERIC = false, bool SWZ = false>
+__device__ void fft_lds(c2_t<T>* buf, const FftDesc<T>& d) {
+  int ns = 1;
+  if constexpr (SWZ) {
+    if ((d.n & 15) == 0 && d.nstages > 1) {
+      const int last = d.nstages - 1;
+      fft_stage_r<T, NT, NMAX, GENERIC, false, true>(buf, d, d.radix[0], ns);
+      ns *= d.radix[0];
+      for (int s = 1; s < last; ++s) {
+        fft_stage_r<T, NT, NMAX, GENERIC, true, true>(buf, d, d.radix[s], ns);
+        ns *= d.radix[s];
+      }
+      fft_stage_r<T, NT, NMAX, GENERIC, true, false>(buf, d, d.radix[last], ns);
+      return;
+    }
+  }
+  for (int s = 0; s < d.nstages; ++s) {
+    fft_stage_r<T, NT, NMAX, GENERIC, false, false>(buf, d, d.radix[s], ns);
+    ns *= d.radix[s];
+  }
+}
+
 template <typename T>
 class FftLength {
  public:

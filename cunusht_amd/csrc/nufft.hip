@@ -72,7 +72,7 @@ __global__ __launch_bounds__(NT) void k_theta2(ThetaArgs<T> a) {
     buf[r] = T2{(T)v.x, (T)v.y};
   }
   __syncthreads();
-  fft_lds<T, NT, NMAX>(buf, a.ftd);
+  fft_lds<T, NT, NMAX, false, true>(buf, a.ftd);
   // deconvolve, pad to nf (conjugated for the inverse transform)
   constexpr int E = (NMAX + NT - 1) / NT;
   T2 v[E];
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(NT) void k_theta2(ThetaArgs<T> a) {
     if (k < a.nf) buf[k] = v[i];
   }
   __syncthreads();
-  fft_lds<T, NT, NMAX>(buf, a.fnf);
+  fft_lds<T, NT, NMAX, false, true>(buf, a.fnf);
   // rows of the kept band; spin 0 also fills the conjugate column
   for (int q = threadIdx.x; q < a.nrow; q += NT) {
     int t = a.row_lo + q;
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(NT) void k_phi2(PhiArgs<T> a) {
     buf[k] = v;
   }
   __syncthreads();
-  fft_lds<T, NT, NMAX>(buf, a.fnf);
+  fft_lds<T, NT, NMAX, false, true>(buf, a.fnf);
   T2* grow = a.grid + (long)q * a.nf;
   for (int k = threadIdx.x; k < a.nf; k += NT) {
     T2 s = buf[k];
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(NT) void k_phi1(PhiArgs<T> a) {
   const T2* grow = a.grid + (long)q * a.nf;
   for (int k = threadIdx.x; k < a.nf; k += NT) buf[k] = grow[k];
   __syncthreads();
-  fft_lds<T, NT, NMAX>(buf, a.fnf);
+  fft_lds<T, NT, NMAX, false, true>(buf, a.fnf);
   T2* trow = a.Tbuf + (long)q * a.nm;
   for (int i = threadIdx.x; i < a.nm; i += NT) {
     int m = i - a.lmax;
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(NT) void k_theta1(ThetaArgs<T> a) {
     buf[t] = a.Tout[(long)q * a.nm + m + a.lmax];
   }
   __syncthreads();
-  fft_lds<T, NT, NMAX>(buf, a.fnf);
+  fft_lds<T, NT, NMAX, false, true>(buf, a.fnf);
   constexpr int E = (NMAX + NT - 1) / NT;
   T2 v[E];
   const double fm = a.deconv[am] / a.ntd;
@@ -212,7 +212,7 @@ __global__ __launch_bounds__(NT) void k_theta1(ThetaArgs<T> a) {
     if (r < a.ntd) buf[r] = v[i];
   }
   __syncthreads();
-  fft_lds<T, NT, NMAX>(buf, a.ftd);
+  fft_lds<T, NT, NMAX, false, true>(buf, a.ftd);
   // fold back onto the ntheta rings (conjugate back from the inverse trick)
   double2* g = const_cast<double2*>(a.G) + (long)row * a.gsm;
   for (int j = threadIdx.x; j < a.ntheta; j += NT) {
