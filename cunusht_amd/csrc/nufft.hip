@@ -232,7 +232,7 @@ template <typename T>
 struct InterpArgs {
   int nf, w, row_lo, nrow, spin, tr, tc, ntc;
   T beta;
-  const T* kpoly;  // [(deg + 1)][w - 2] interior-cell kernel polynomials
+  const T* kpoly;  // [(deg + 1)][w] kernel polynomials, one per footprint cell
   int deg;
   long npts;
   const double2* pts;
@@ -245,32 +245,26 @@ struct InterpArgs {
 };
 
 // Kernel weights of one point along both axes for a compile-time width W:
-// k[i] = psi((f + i) 2/W) with f = (first cell) - (position) in [-W/2, -W/2 + 1).  Interior
-// cells come from the plan's Horner polynomials in u = f + W/2 - 1/2 (coefficients are
-// wave-uniform), the two outer cells from the exact kernel.
+// k[i] = psi((f + i) 2/W) with f = (first cell) - (position) in [-W/2, -W/2 + 1), from the plan's
+// Horner polynomials in u = f + W/2 - 1/2 (one per cell, coefficients wave-uniform).  The two outer
+// cells hold the square-root branch point of psi at |z| = 1, where a polynomial converges only
+// algebraically; there psi is below e^{-beta} * beta (~1e-8 at eps 1e-7) and the plan's fit still meets
+// max(eps/100, 1e-15) with the same degree as the interior cells (checked at plan time on 513 samples
+// per cell), so no exp / sqrt is evaluated per point.
 template <typename T, int W>
 __device__ __forceinline__ void ker_weights(double ft, double fp, T beta, const T* __restrict__ cp, int deg,
                                             T (&kt)[W], T (&kp)[W]) {
-  const T inv = T(2) / T(W);
-  kt[0] = WorkT<T>::ker((T)ft * inv, beta);
-  kp[0] = WorkT<T>::ker((T)fp * inv, beta);
-  if constexpr (W > 1) {
-    kt[W - 1] = WorkT<T>::ker((T)(ft + (W - 1)) * inv, beta);
-    kp[W - 1] = WorkT<T>::ker((T)(fp + (W - 1)) * inv, beta);
-  }
-  if constexpr (W > 2) {
-    constexpr int NI = W - 2;
-    const T ut = (T)(ft + 0.5 * W - 0.5), up = (T)(fp + 0.5 * W - 0.5);
-    const T* c = cp + (long)deg * NI;
+  (void)beta;
+  const T ut = (T)(ft + 0.5 * W - 0.5), up = (T)(fp + 0.5 * W - 0.5);
+  const T* c = cp + (long)deg * W;
 #pragma unroll
-    for (int i = 0; i < NI; ++i) { kt[i + 1] = c[i]; kp[i + 1] = c[i]; }
-    for (int k = deg - 1; k >= 0; --k) {
-      c -= NI;
+  for (int i = 0; i < W; ++i) { kt[i] = c[i]; kp[i] = c[i]; }
+  for (int k = deg - 1; k >= 0; --k) {
+    c -= W;
 #pragma unroll
-      for (int i = 0; i < NI; ++i) {
-        kt[i + 1] = fma(kt[i + 1], ut, c[i]);
-        kp[i + 1] = fma(kp[i + 1], up, c[i]);
-      }
+    for (int i = 0; i < W; ++i) {
+      kt[i] = fma(kt[i], ut, c[i]);
+      kp[i] = fma(kp[i], up, c[i]);
     }
   }
 }
@@ -761,7 +755,7 @@ NufftParams NufftParams::choose(int lmax, int spin, double eps, double sigma_hin
 // sample (evaluated by Horner in the working precision) is below `target`.
 static void fit_kernel_poly(int w, double beta, double target, bool single, int& deg, double& err,
                             std::vector<double>& coef) {
-  const int ni = w - 2;
+  const int ni = w;  // every cell of the footprint, the two outer ones included (see ker_weights)
   deg = 0; err = 0; coef.clear();
   if (ni <= 0) return;
   auto psi = [&](long double z) -> long double {
@@ -774,7 +768,7 @@ static void fit_kernel_poly(int w, double beta, double target, bool single, int&
     std::vector<double> c((size_t)n * ni);
     std::vector<long double> un(n);
     for (int j = 0; j < n; ++j) un[j] = 0.5L * std::cos((long double)M_PI * (j + 0.5L) / n);
-    for (int i = 1; i <= ni; ++i) {
+    for (int i = 0; i < ni; ++i) {
       // Vandermonde solve in long double (Gaussian elimination, partial pivoting)
       std::vector<long double> A((size_t)n * (n + 1));
       for (int r = 0; r < n; ++r) {
@@ -799,20 +793,20 @@ static void fit_kernel_poly(int w, double beta, double target, bool single, int&
         for (int k = r + 1; k < n; ++k) v -= A[(size_t)r * (n + 1) + k] * x[k];
         x[r] = v / A[(size_t)r * (n + 1) + r];
       }
-      for (int k = 0; k < n; ++k) c[(size_t)k * ni + (i - 1)] = (double)x[k];
+      for (int k = 0; k < n; ++k) c[(size_t)k * ni + i] = (double)x[k];
     }
     double e = 0;
-    for (int i = 1; i <= ni; ++i)
+    for (int i = 0; i < ni; ++i)
       for (int q = 0; q <= 512; ++q) {
         const double u = -0.5 + q / 512.0;
         double v;
         if (single) {
-          float fv = (float)c[(size_t)d * ni + (i - 1)];
-          for (int k = d - 1; k >= 0; --k) fv = std::fma(fv, (float)u, (float)c[(size_t)k * ni + (i - 1)]);
+          float fv = (float)c[(size_t)d * ni + i];
+          for (int k = d - 1; k >= 0; --k) fv = std::fma(fv, (float)u, (float)c[(size_t)k * ni + i]);
           v = fv;
         } else {
-          v = c[(size_t)d * ni + (i - 1)];
-          for (int k = d - 1; k >= 0; --k) v = std::fma(v, u, c[(size_t)k * ni + (i - 1)]);
+          v = c[(size_t)d * ni + i];
+          for (int k = d - 1; k >= 0; --k) v = std::fma(v, u, c[(size_t)k * ni + i]);
         }
         e = std::max(e, (double)std::fabs((long double)v - psi((u + 0.5L + i - 0.5L * w) * 2.0L / w)));
       }
