@@ -305,6 +305,8 @@ int nusht_plan_set_pts_async(nusht_plan_t p, const double* theta, const double* 
   });
 }
 
+}  // extern "C"
+
 static void synth_batch(nusht_plan_t p, int nmaps, const double* alm, long alm_stride, double* out0, double* out1,
                         long out_stride, const double* gamma, hipStream_t st) {
   const long gmap = (long)p->nmrows * p->np.ntheta;
@@ -360,6 +362,8 @@ void op_lenmap2gclm(nusht_plan_t p, nusht_ringplan_t ptg, const double* lenmap0,
 }
 
 }  // namespace
+
+extern "C" {
 
 int nusht_nusht2d2(nusht_plan_t p, const double* alm, double* out0, double* out1, const double* gamma,
                    nusht_stream_t s) {
@@ -473,6 +477,8 @@ int nusht_nufft_spread(nusht_plan_t p, const double* in0, const double* in1, con
   });
 }
 
+}  // extern "C"
+
 // ------------------------------------------------------------------ batches of independent transforms
 namespace {
 
@@ -545,6 +551,8 @@ int find_root(std::vector<int>& par, int a) {
 }
 
 }  // namespace
+
+extern "C" {
 
 int nusht_batch_execute(const nusht_job* jobs, int njobs, nusht_stream_t s) {
   return guard([&] {
