@@ -144,7 +144,7 @@ def cpu_baseline(lmax_sample: int, lmax: int, eps: float):
 def serve_headline(args, L, eps, gclms, dlms, rng):
     """Replay spin-2 gclm2lenmap requests through the continuous-batching server: Poisson arrivals at
     args.serve_rate, four deflections each shared by a quarter of the requests (same-deflection requests
-    queued together are batched, up to 4), two resident plans.  Returns throughput and latency."""
+    queued together are batched, up to 8), two resident plans.  Returns throughput and latency."""
     import torch
     from cunusht_amd.serving import LensingServer, Request, poisson_arrivals, serve
     from cunusht_amd.utils import synalm
@@ -155,7 +155,7 @@ def serve_headline(args, L, eps, gclms, dlms, rng):
         k = len(defl)
         m = torch.as_tensor(np.concatenate([np.full(L + 1 - mm, mm) for mm in range(L + 1)]), device="cuda")
         defl.append(dlms[0] * torch.exp(1j * 0.3 * k * m))
-    server = LensingServer(hbm_budget_bytes=160 << 30, plans_per_key=2, max_batch=4, geometry=geo)
+    server = LensingServer(hbm_budget_bytes=160 << 30, plans_per_key=2, max_batch=8, geometry=geo)
     key = (L, 2, float(eps), 2)
 
     def requests(n, seed):
@@ -169,7 +169,7 @@ def serve_headline(args, L, eps, gclms, dlms, rng):
     n = args.serve_requests
     st = serve(server, requests(n, 2), poisson_arrivals(n, args.serve_rate, seed=3))
     return {"maps_per_s": st["requests_per_s"], "requests": n, "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"],
-            "arrival_rate_per_s": args.serve_rate, "deflections": ndefl, "max_batch": 4, "plans": st["plans"],
+            "arrival_rate_per_s": args.serve_rate, "deflections": ndefl, "max_batch": 8, "plans": st["plans"],
             "plan_bytes": st["plan_bytes"], "wall_s": st["wall_s"]}
 
 
