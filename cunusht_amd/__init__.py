@@ -2,6 +2,8 @@
 
 Drop-in for cunuSHT's GPU backend: get_transformer(backend='GPU') returns the operator class with
 nusht2d2, nusht2d1, gclm2lenmap, lenmap2gclm and dlm2pointing (cunusht/cunusht.py:63-98).
+get_transformer(backend='CPU') returns the host operator class with the signatures of the reference's
+CPU_DUCCnufft_transformer (cunusht/deflection/CPU_nufft_transformer.py:181-558).
 """
 from __future__ import annotations
 
@@ -14,15 +16,22 @@ __all__ = ["get_transformer", "get_geom", "Geom", "Alm", "almxfl", "synalm", "go
 
 
 def get_transformer(backend: str, solver: str = "default", mode: str = "nuFFT"):
-    """Returns the transformer class for `backend` (cunusht/cunusht.py:63-82).
+    """Returns the transformer class for `backend` (cunusht/cunusht.py:63-82), mode 'nuFFT'.
 
-    backend 'GPU' (solver 'default' / 'cufinufft' / 'amd'), mode 'nuFFT'.  The reference's CPU
-    backend (ducc0 / lenspyx) is not part of this package.
+    backend 'GPU': solver 'default' / 'cufinufft' / 'amd' -> GPU_nusht_transformer (libnusht_amd.so).
+    backend 'CPU': solver 'default' / 'duccnufft' / 'amd' -> CPU_nusht_transformer (libnusht_cpu.so), the
+    algorithm and signatures of the reference's CPU_DUCCnufft_transformer.  The reference's other CPU solvers
+    (lenspyx, finufft) wrap third-party packages and are not provided.
     """
-    if backend not in ("GPU",):
+    if backend not in ("GPU", "CPU"):
         raise AssertionError(f"{backend} backend not found")
     if mode not in ("nuFFT",):
         raise AssertionError(f"{mode} mode not found")
+    if backend == "CPU":
+        if solver not in ("default", "duccnufft", "amd"):
+            raise AssertionError(f"Solver not found: {solver}")
+        from .cpu_transformer import CPU_nusht_transformer
+        return CPU_nusht_transformer
     if solver not in ("default", "cufinufft", "amd"):
         raise AssertionError(f"Solver not found: {solver}")
     from .transformer import GPU_nusht_transformer

@@ -748,11 +748,12 @@ NufftParams NufftParams::choose(int lmax, int spin, double eps, double sigma_hin
 
 // ------------------------------------------------------------------ plan
 // Piecewise-polynomial ES kernel.  A point at fractional offset t in [0, 1) from its first cell
-// needs psi(z_i), z_i = (t + i - w/2) 2/w, i = 0..w-1.  For the interior cells i = 1..w-2 the
-// kernel is analytic in t and psi(z_i) = sum_k c[k][i-1] u^k, u = t - 1/2, is interpolated at
-// degree + 1 Chebyshev nodes; the two outer cells hold the square-root branch point at |z| = 1
-// and are evaluated exactly.  The degree is the lowest in [2, 24] whose max error on a dense
-// sample (evaluated by Horner in the working precision) is below `target`.
+// needs psi(z_i), z_i = (t + i - w/2) 2/w, i = 0..w-1.  Every cell gets its own polynomial
+// psi(z_i) = sum_k c[k][i] u^k, u = t - 1/2, interpolated at degree + 1 Chebyshev nodes.  The two
+// outer cells hold the square-root branch point at |z| = 1, where psi is already below
+// e^{-beta (1 - sqrt(1 - (1 - 2/w)^2))} ~ eps, so their fit error stays below the target as well.
+// The degree is the lowest in [2, 24] whose max error on a dense sample (evaluated by Horner in
+// the working precision) over all cells is below `target`.
 static void fit_kernel_poly(int w, double beta, double target, bool single, int& deg, double& err,
                             std::vector<double>& coef) {
   const int ni = w;  // every cell of the footprint, the two outer ones included (see ker_weights)

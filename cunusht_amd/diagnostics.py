@@ -14,15 +14,10 @@ values at the points).  The fused pipeline never forms these arrays; they are fo
 """
 from __future__ import annotations
 
-import ctypes
 import json
 import os
 import time
 
-import numpy as np
-import torch
-
-from . import _lib
 from .geometry import Geom, good_size
 
 # library stage -> the reference's timer key (its decorated function names, helper_GPU.py:13)
@@ -55,8 +50,24 @@ class Timer:
         return "\n".join(f"{k:>20s}: {v * 1e3:9.3f} ms" for k, v in self.keys.items())
 
 
+def timing_root() -> str:
+    return os.environ.get("NUSHT_TIMING_DIR", os.path.join(os.getcwd(), "timings"))
+
+
+def dump_timer(transformer, name: str, fname: str) -> str:
+    """Writes transformer.timer.keys as JSON to <root>/<class>/<name>/<fname>; returns the path."""
+    d = os.path.join(timing_root(), type(transformer).__name__, name)
+    os.makedirs(d, exist_ok=True)
+    path = os.path.join(d, fname)
+    transformer.timer.dumpjson(path)
+    return path
+
+
 def timed(transformer, name: str, lmax: int, runid, GPUdesc: str, fn):
     """Run fn() with stage timing and store / dump the timer keys."""
+    import torch
+
+    from . import _lib
     _lib.timing_reset()
     _lib.timing_enable(True)
     try:
@@ -74,11 +85,8 @@ def timed(transformer, name: str, lmax: int, runid, GPUdesc: str, fn):
         keys[k] = keys.get(k, 0.0) + ms / 1e3
     keys[name] = total
     transformer.timer.keys = keys
-    root = os.environ.get("NUSHT_TIMING_DIR", os.path.join(os.getcwd(), "timings"))
-    d = os.path.join(root, type(transformer).__name__, name)
-    os.makedirs(d, exist_ok=True)
-    transformer.timer.dumpjson(os.path.join(
-        d, "lmax{}_epsilon{}_run{:02d}_{}".format(lmax, transformer.epsilon, int(runid or 0), GPUdesc)))
+    dump_timer(transformer, name,
+               "lmax{}_epsilon{}_run{:02d}_{}".format(lmax, transformer.epsilon, int(runid or 0), GPUdesc))
     print(15 * "- " + "Timing {}: {:.3f} seconds".format(name, total) + 15 * "- ")
     return res
 
@@ -88,8 +96,11 @@ def reference_car_sizes(lmax: int):
     return (good_size(lmax + 2) + 3) // 4 * 4, 2 * good_size(lmax + 1)
 
 
-def debug_forward(transformer, alm: torch.Tensor, spin: int, ncomp: int, values) -> dict:
+def debug_forward(transformer, alm, spin: int, ncomp: int, values) -> dict:
     """The reference's forward intermediates on its CAR grid (see module docstring)."""
+    import torch
+
+    from . import _lib
     from .transformer import RingSHT, _ptr, _stream
     L = transformer.lmax
     nt, nph = reference_car_sizes(L)
