@@ -42,8 +42,8 @@ _dp = ctypes.POINTER(ctypes.c_double)
 _LIB = None
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libnusht_cpu.so")
 
-SYMBOLS = ("nusht_cpu_version", "nusht_cpu_synth_rings", "nusht_cpu_adjoint_rings", "nusht_cpu_interp",
-           "nusht_cpu_spread", "nusht_cpu_pointing")
+SYMBOLS = ("nusht_cpu_version", "nusht_cpu_synth_rings", "nusht_cpu_adjoint_rings", "nusht_cpu_es_kernel_info",
+           "nusht_cpu_interp", "nusht_cpu_spread", "nusht_cpu_pointing")
 
 
 def load():
@@ -57,8 +57,9 @@ def load():
         lib.nusht_cpu_version.argtypes = []
         lib.nusht_cpu_synth_rings.argtypes = [i, i, i, _dp, i, _dp, _dp, i]
         lib.nusht_cpu_adjoint_rings.argtypes = [i, i, i, _dp, i, _dp, _dp, i]
-        lib.nusht_cpu_interp.argtypes = [i, i, i, i, i, d, _dp, i64, _dp, _dp, _dp, i]
-        lib.nusht_cpu_spread.argtypes = [i, i, i, i, i, d, _dp, i64, _dp, _dp, _dp, i]
+        lib.nusht_cpu_es_kernel_info.argtypes = [i, d, d, ctypes.POINTER(i), _dp]
+        lib.nusht_cpu_interp.argtypes = [ctypes.c_void_p, _dp, i64, _dp, _dp, i, _dp, i, _dp, i]
+        lib.nusht_cpu_spread.argtypes = [ctypes.c_void_p, _dp, i64, _dp, _dp, i, _dp, i, _dp, i]
         lib.nusht_cpu_pointing.argtypes = [i64, _dp, _dp, _dp, _dp, _dp, _dp, _dp, i]
         for name in SYMBOLS:
             getattr(lib, name).restype = ctypes.c_int
@@ -162,13 +163,30 @@ def phi_adjoint_rings(M: np.ndarray, lmax: int, spin: int, phi0, nthreads=None) 
 
 
 # ---------------------------------------------------------------------------------------- ES nuFFT
+class _Grid(ctypes.Structure):
+    """nusht_cpu_grid (include/nusht_cpu.h)."""
+    _fields_ = [("nf1", ctypes.c_int), ("nf2", ctypes.c_int), ("row_lo", ctypes.c_int), ("nrow", ctypes.c_int),
+                ("w", ctypes.c_int), ("beta", ctypes.c_double), ("ktol", ctypes.c_double)]
+
+
+def _runs(dst):
+    """(src, dst, length) runs of consecutive entries of the index map src -> dst[src]."""
+    dst = np.asarray(dst)
+    cut = np.nonzero(np.diff(dst) != 1)[0] + 1
+    starts = np.concatenate([[0], cut])
+    ends = np.concatenate([cut, [dst.size]])
+    return [(int(a), int(dst[a]), int(b - a)) for a, b in zip(starts, ends)]
+
+
 class EsGrid:
     """ES-kernel nuFFT between 2D Fourier modes (theta modes k = -lmax..lmax, phi modes of the spin) and points.
 
     Kernel parameters by the published ES method (Barnett, Magland & af Klinteberg, SIAM J. Sci. Comput. 41
     (2019)): upsampling sigma = 2 for eps < 1e-8 else 1.25, width w = ceil(ln(1/eps) / (pi sqrt(1 - 1/sigma))),
     beta = 0.97 pi w (1 - 1/(2 sigma)); fine sizes even and 2,3,5,7,11-smooth, at least 2 w; deconvolution by
-    the kernel's Fourier transform, integrated by Gauss-Legendre quadrature."""
+    the kernel's Fourier transform, integrated by Gauss-Legendre quadrature; kernel values from per-cell
+    polynomials accurate to max(eps / 100, 1e-15).  Spin 0 carries the phi modes m = 0..lmax shifted by
+    c = lmax // 2 (m - c is centred), the phase e^{i c phi} is applied at the points."""
 
     def __init__(self, lmax: int, spin: int, eps: float):
         if not (0.0 < eps < 1.0):
@@ -195,9 +213,11 @@ class EsGrid:
         hi = int(math.ceil(0.5 * self.nf1 - 0.5 * w)) + w
         self.nrow = min(self.nf1, hi - lo + 1)
         self.row_lo = 0 if self.nrow == self.nf1 else lo
-        self.rows = (self.row_lo + np.arange(self.nrow)) % self.nf1
-        self.kidx = self.ks % self.nf1
-        self.midx = mm % self.nf2
+        self.grid = _Grid(self.nf1, self.nf2, self.row_lo, self.nrow, w, self.beta, max(eps / 100.0, 1e-15))
+        # index maps as contiguous runs: theta modes -> torus rows, window rows -> torus rows, phi modes -> columns
+        self.k_runs = _runs(self.ks % self.nf1)
+        self.row_runs = _runs((self.row_lo + np.arange(self.nrow)) % self.nf1)
+        self.m_runs = _runs(mm % self.nf2)
         xq, wq = np.polynomial.legendre.leggauss(4 * w + 60)
         psi = wq * np.exp(self.beta * (np.sqrt(np.maximum(0.0, 1.0 - xq * xq)) - 1.0))
 
@@ -206,8 +226,11 @@ class EsGrid:
 
         self.dec = np.outer(inv_hat(self.ks, self.nf1), inv_hat(mm, self.nf2))
 
-    def _grid_args(self):
-        return self.nf1, self.nf2, self.row_lo, self.nrow, self.w, self.beta
+    def kernel_info(self):
+        """(degree, max error) of the kernel polynomials."""
+        deg, err = ctypes.c_int(), ctypes.c_double()
+        _call("nusht_cpu_es_kernel_info", self.w, self.beta, self.grid.ktol, ctypes.byref(deg), ctypes.byref(err))
+        return deg.value, err.value
 
     def check_points(self, theta, phi):
         theta, phi = _f64(theta), _f64(phi)
@@ -219,33 +242,52 @@ class EsGrid:
             raise ValueError("point longitudes must be finite")
         return theta, phi
 
-    def u2nu(self, F: np.ndarray, theta, phi, nthreads=None) -> np.ndarray:
-        """sum_{k,m} F[k, m] e^{i (k theta + (m - c) phi)} at the points (complex, within eps)."""
-        nth = _nthreads(nthreads)
-        A = np.zeros((self.nf1, self.ms.size), dtype=np.complex128)
-        A[self.kidx] = F * self.dec
-        A = sfft.ifft(A, axis=0, norm="forward", workers=nth, overwrite_x=True)
+    def _window(self, A, nth):
+        """Torus rows (nf1, nmodes) of the theta transform -> the window (nrow, nf2) over phi."""
         W = np.zeros((self.nrow, self.nf2), dtype=np.complex128)
-        W[:, self.midx] = A[self.rows]
-        W = sfft.ifft(W, axis=1, norm="forward", workers=nth, overwrite_x=True)
-        W = np.ascontiguousarray(W)
+        for rs, rd, rn in self.row_runs:
+            for ms, md, mn in self.m_runs:
+                W[rs:rs + rn, md:md + mn] = A[rd:rd + rn, ms:ms + mn]
+        return sfft.ifft(W, axis=1, norm="forward", workers=nth, overwrite_x=True)
+
+    def u2nu(self, F: np.ndarray, theta, phi, nthreads=None, gamma=None, spin_rot: int = 0) -> np.ndarray:
+        """e^{i (c phi + spin_rot gamma)} sum_{k,m} F[k, m] e^{i (k theta + (m - c) phi)} at the points (within
+        eps).  F is (2 lmax + 1, nmodes) and may be overwritten."""
+        nth = _nthreads(nthreads)
+        F *= self.dec
+        A = np.zeros((self.nf1, self.ms.size), dtype=np.complex128)
+        for s0, d0, n in self.k_runs:
+            A[d0:d0 + n] = F[s0:s0 + n]
+        A = sfft.ifft(A, axis=0, norm="forward", workers=nth, overwrite_x=True)
+        W = self._window(A, nth)
+        del A
         out = np.empty(theta.size, dtype=np.complex128)
-        _call("nusht_cpu_interp", *self._grid_args(), _p(W.view(np.float64)), theta.size, _p(theta), _p(phi),
-              _p(out.view(np.float64)), nth)
+        g = None if gamma is None or not spin_rot else _f64(gamma)
+        _call("nusht_cpu_interp", ctypes.byref(self.grid), _p(W.view(np.float64)), theta.size, _p(theta), _p(phi),
+              self.shift, _p(g), int(spin_rot), _p(out.view(np.float64)), nth)
         return out
 
-    def nu2u(self, values: np.ndarray, theta, phi, nthreads=None) -> np.ndarray:
-        """Exact adjoint of u2nu: F'[k, m] = sum_p values_p e^{-i (k theta_p + (m - c) phi_p)} (within eps)."""
+    def nu2u(self, values: np.ndarray, theta, phi, nthreads=None, gamma=None, spin_rot: int = 0) -> np.ndarray:
+        """Exact adjoint of u2nu: F'[k, m] = sum_p e^{-i (k theta_p + (m - c) phi_p)} e^{-i (c phi_p + spin_rot
+        gamma_p)} values_p (within eps)."""
         nth = _nthreads(nthreads)
         v = _c128(values)
         W = np.zeros((self.nrow, self.nf2), dtype=np.complex128)
-        _call("nusht_cpu_spread", *self._grid_args(), _p(v.view(np.float64)), v.size, _p(theta), _p(phi),
-              _p(W.view(np.float64)), nth)
+        g = None if gamma is None or not spin_rot else _f64(gamma)
+        _call("nusht_cpu_spread", ctypes.byref(self.grid), _p(v.view(np.float64)), v.size, _p(theta), _p(phi),
+              self.shift, _p(g), int(spin_rot), _p(W.view(np.float64)), nth)
         W = sfft.fft(W, axis=1, workers=nth, overwrite_x=True)
         T = np.zeros((self.nf1, self.ms.size), dtype=np.complex128)
-        T[self.rows] = W[:, self.midx]
+        for rs, rd, rn in self.row_runs:  # window rows are distinct torus rows
+            for ms, md, mn in self.m_runs:
+                T[rd:rd + rn, ms:ms + mn] = W[rs:rs + rn, md:md + mn]
+        del W
         T = sfft.fft(T, axis=0, workers=nth, overwrite_x=True)
-        return T[self.kidx] * self.dec
+        out = np.empty((self.ks.size, self.ms.size), dtype=np.complex128)
+        for s0, d0, n in self.k_runs:
+            out[s0:s0 + n] = T[d0:d0 + n]
+        out *= self.dec
+        return out
 
 
 # ---------------------------------------------------------------------------------------- the operator
@@ -308,27 +350,32 @@ class CPU_nusht_transformer:
         return _f64(loc[:, 0]), _f64(loc[:, 1])
 
     def _theta_coeffs(self, G, lmax, spin, nthreads):
-        """Doubling over theta of every m row and its normalised theta FFT: F[k, m], k = -lmax..lmax."""
+        """Doubling over theta of every m row and its normalised theta FFT: F[k, m], k = -lmax..lmax, from the
+        CC ring coefficients G (rows m, nt rings): D[r >= nt] = (-1)^(s + m) G[2 nt - 2 - r]."""
         nt = G.shape[1]
         n1 = 2 * nt - 2
         ms = _ms(lmax, spin)
-        sg = np.where((spin + ms) % 2 == 0, 1.0, -1.0)[:, None]
-        D = np.empty((G.shape[0], n1), dtype=np.complex128)
-        D[:, :nt] = G
-        D[:, nt:] = sg * G[:, nt - 2:0:-1]
-        D = sfft.fft(D, axis=1, norm="forward", workers=_nthreads(nthreads), overwrite_x=True)
-        return np.ascontiguousarray(D[:, np.arange(-lmax, lmax + 1) % n1].T)
+        sg = np.where((spin + ms) % 2 == 0, 1.0, -1.0)[None, :]
+        D = np.empty((n1, G.shape[0]), dtype=np.complex128)
+        D[:nt] = G.T
+        np.multiply(D[nt - 2:0:-1], sg, out=D[nt:])
+        D = sfft.fft(D, axis=0, norm="forward", workers=_nthreads(nthreads), overwrite_x=True)
+        F = np.empty((2 * lmax + 1, G.shape[0]), dtype=np.complex128)
+        F[:lmax] = D[n1 - lmax:]
+        F[lmax:] = D[:lmax + 1]
+        return F
 
     def _theta_coeffs_adjoint(self, Fp, lmax, spin, nt, nthreads):
         n1 = 2 * nt - 2
         ms = _ms(lmax, spin)
-        sg = np.where((spin + ms) % 2 == 0, 1.0, -1.0)[:, None]
-        D = np.zeros((ms.size, n1), dtype=np.complex128)
-        D[:, np.arange(-lmax, lmax + 1) % n1] = Fp.T
-        D = sfft.ifft(D, axis=1, workers=_nthreads(nthreads), overwrite_x=True)
-        Gp = D[:, :nt].copy()
-        Gp[:, 1:nt - 1] += sg * D[:, n1 - 1:nt - 1:-1]
-        return Gp
+        sg = np.where((spin + ms) % 2 == 0, 1.0, -1.0)[None, :]
+        D = np.zeros((n1, ms.size), dtype=np.complex128)
+        D[n1 - lmax:] = Fp[:lmax]
+        D[:lmax + 1] = Fp[lmax:]
+        D = sfft.ifft(D, axis=0, workers=_nthreads(nthreads), overwrite_x=True)
+        Gp = D[:nt].copy()
+        Gp[1:nt - 1] += sg * D[n1 - 1:nt - 1:-1]
+        return Gp.T
 
     def _cc_theta(self, lmax):
         nt = cc_sizes(lmax)[0]
@@ -365,16 +412,12 @@ class CPU_nusht_transformer:
                                                     alm=out, apply_weights=apply_weights, mode=mode)
 
     # ------------------------------------------------------------------ general (nu) synthesis
-    def synthesis_general(self, pointmap, lmax, mmax, alm, loc, spin, epsilon, nthreads, mode, verbose):
-        """alm -> complex values at loc (npts, 2) (CPU_nufft_transformer.py:390-405): spin 0 gives the real
-        field (imaginary part 0), spin > 0 gives Q + iU.  mode 'GRAD_ONLY' or a single component is E only."""
-        theta, phi = self._loc(loc)
-        a = _alm_full(alm, lmax, mmax)
-        if mode == "GRAD_ONLY" and spin > 0:
-            a = a[:1]
+    def _synth_general(self, a, lmax, spin, theta, phi, eps, nthreads, gamma=None):
+        """alm (ncomp, nalm(lmax, lmax)) -> complex values at the points, times e^{i spin gamma} when gamma is
+        given (spin 0: real field with imaginary part 0)."""
         if not (a.shape[0] == 1 or (spin > 0 and a.shape[0] == 2)):
             raise ValueError(f"spin {spin} takes 1 or 2 alm components, got {a.shape[0]}")
-        grid = self._grid(lmax, spin, epsilon)
+        grid = self._grid(lmax, spin, eps)
         theta, phi = grid.check_points(theta, phi)
         t = time.perf_counter()
         G = synth_rings(a, lmax, spin, self._cc_theta(lmax), nthreads)
@@ -385,11 +428,41 @@ class CPU_nusht_transformer:
         if spin == 0:
             G[1:] *= 2.0
         F = self._theta_coeffs(G, lmax, spin, nthreads)
+        del G
         t = self._tick("doubling", t)
-        u = grid.u2nu(F, theta, phi, nthreads)
+        u = grid.u2nu(F, theta, phi, nthreads, gamma=gamma, spin_rot=spin if gamma is not None else 0)
         if spin == 0:
-            u = (u * np.exp(1j * grid.shift * phi)).real.astype(np.complex128)
+            u = u.real.astype(np.complex128)
         self._tick("nuFFT2d2", t)
+        return u
+
+    def _adjoint_general(self, v, lmax, spin, ncomp, theta, phi, eps, nthreads, gamma=None):
+        """Exact adjoint of _synth_general: complex values (spin 0: real part used) -> (ncomp, nalm(lmax, lmax))."""
+        grid = self._grid(lmax, spin, eps)
+        theta, phi = grid.check_points(theta, phi)
+        v = np.asarray(v).reshape(-1)
+        if v.size != theta.size:
+            raise ValueError(f"pointmap has {v.size} values for {theta.size} points")
+        t = time.perf_counter()
+        u = np.real(v).astype(np.complex128) if spin == 0 else _c128(v)
+        Fp = grid.nu2u(u, theta, phi, nthreads, gamma=gamma, spin_rot=spin if gamma is not None else 0)
+        t = self._tick("nuFFT2d1", t)
+        th = self._cc_theta(lmax)
+        Gp = self._theta_coeffs_adjoint(Fp, lmax, spin, th.size, nthreads)
+        del Fp
+        t = self._tick("adjointdoubling", t)
+        res = adjoint_rings(Gp, lmax, spin, ncomp, th, nthreads)
+        self._tick("adjointsynthesis", t)
+        return res
+
+    def synthesis_general(self, pointmap, lmax, mmax, alm, loc, spin, epsilon, nthreads, mode, verbose):
+        """alm -> complex values at loc (npts, 2) (CPU_nufft_transformer.py:390-405): spin 0 gives the real
+        field (imaginary part 0), spin > 0 gives Q + iU.  mode 'GRAD_ONLY' or a single component is E only."""
+        theta, phi = self._loc(loc)
+        a = _alm_full(alm, lmax, mmax)
+        if mode == "GRAD_ONLY" and spin > 0:
+            a = a[:1]
+        u = self._synth_general(a, lmax, spin, theta, phi, epsilon, nthreads)
         if self.execmode == "debug":
             self.ret["nuFFT2d2"] = u.copy()
         return u
@@ -398,26 +471,9 @@ class CPU_nusht_transformer:
         """Exact adjoint of synthesis_general (CPU_nufft_transformer.py:410-427): complex values -> alm
         (ncomp, nalm(lmax, mmax)); ncomp = 1 for spin 0 or mode 'GRAD_ONLY', else 2."""
         theta, phi = self._loc(loc)
-        grid = self._grid(lmax, spin, epsilon)
-        theta, phi = grid.check_points(theta, phi)
-        v = np.asarray(pointmap).reshape(-1)
-        if v.size != theta.size:
-            raise ValueError(f"pointmap has {v.size} values for {theta.size} points")
         ncomp = 1 if (spin == 0 or mode == "GRAD_ONLY") else 2
-        t = time.perf_counter()
-        if spin == 0:
-            u = np.real(v).astype(np.float64) * np.exp(-1j * grid.shift * phi)
-        else:
-            u = _c128(v)
-        Fp = grid.nu2u(u, theta, phi, nthreads)
-        t = self._tick("nuFFT2d1", t)
-        th = self._cc_theta(lmax)
-        Gp = self._theta_coeffs_adjoint(Fp, lmax, spin, th.size, nthreads)
-        t = self._tick("adjointdoubling", t)
-        res = adjoint_rings(Gp, lmax, spin, ncomp, th, nthreads)
-        self._tick("adjointsynthesis", t)
-        n = Alm.getsize(lmax, mmax)
-        res = res[:, :n]
+        res = self._adjoint_general(pointmap, lmax, spin, ncomp, theta, phi, epsilon, nthreads)
+        res = res[:, :Alm.getsize(lmax, mmax)]
         if alm is not None:
             alm = np.asarray(alm)
             if alm.size == res.size and alm.dtype in (np.complex128, np.complex64) and alm.flags.writeable:
@@ -446,13 +502,14 @@ class CPU_nusht_transformer:
         return self.deflectionlib.dlm2pointing(dlm, self.lmax if mmax_dlm is None else mmax_dlm, single_prec,
                                                nthreads)
 
-    def _rotate(self, values, polrot, spin, inverse=False):
-        if polrot * spin:
-            g = self.deflectionlib.gamma()
-            if g is None:
-                raise ValueError("polrot needs the rotation angle: pass dlm (not only ptg)")
-            values = values * np.exp((-1j if inverse else 1j) * spin * g)
-        return values
+    def _gamma_for(self, polrot, spin):
+        """The rotation angle when the spin rotation applies, else None."""
+        if not (polrot and spin):
+            return None
+        g = self.deflectionlib.gamma()
+        if g is None:
+            raise ValueError("polrot needs the rotation angle: pass dlm (not only ptg)")
+        return g
 
     def _setup(self, execmode, epsilon, nthreads):
         if execmode in (0, None):
@@ -500,14 +557,17 @@ class CPU_nusht_transformer:
         t0 = time.perf_counter()
         gclm = np.atleast_2d(np.asarray(gclm))
         tp, pp = self._pointing(dlm, ptg, lmax, mmax, nthreads, verbose)
-        loc = np.stack([tp, pp], axis=1)
-        u = self.synthesis_general(lenmap, lmax, mmax, gclm, loc, spin, epsilon, nthreads, _mode(gclm, spin), verbose)
-        t = time.perf_counter()
-        u = self._rotate(u, polrot, spin)
-        self._tick("rotate", t)
-        self._tick("gclm2lenmap", t0)
-        if self.execmode == "debug":
+        a = _alm_full(gclm, lmax, mmax)
+        gam = self._gamma_for(polrot, spin)
+        if self.execmode == "debug":  # the unrotated values too, as the reference's stages return them
+            u = self._synth_general(a, lmax, spin, tp, pp, epsilon, nthreads)
+            self.ret["nuFFT2d2"] = u.copy()
+            if gam is not None:
+                u = u * np.exp(1j * spin * gam)
             self.ret["rotate"] = u.copy()
+        else:
+            u = self._synth_general(a, lmax, spin, tp, pp, epsilon, nthreads, gamma=gam)
+        self._tick("gclm2lenmap", t0)
         res = self._out_real(u.real if spin == 0 else np.array([u.real, u.imag]))
         if lenmap is not None:
             lm = np.asarray(lenmap)
@@ -537,11 +597,9 @@ class CPU_nusht_transformer:
         else:
             lm = lm.reshape(2, -1)
             v = lm[0] + 1j * lm[1]
-        v = self._rotate(v, polrot, spin, inverse=True)
         ncomp = 1 if spin == 0 else (np.atleast_2d(gclm_out).shape[0] if gclm_out is not None else 2)
-        mode = "GRAD_ONLY" if (spin > 0 and ncomp == 1) else "STANDARD"
-        loc = np.stack([tp, pp], axis=1)
-        res = self.adjoint_synthesis_general(lmax, mmax, v, loc, spin, epsilon, nthreads, mode, None, verbose)
+        res = self._adjoint_general(v, lmax, spin, ncomp, tp, pp, epsilon, nthreads,
+                                    gamma=self._gamma_for(polrot, spin))[:, :Alm.getsize(lmax, mmax)]
         res = self._out_complex(res)
         if gclm_out is not None:
             g = np.asarray(gclm_out)

@@ -37,18 +37,34 @@ int nusht_cpu_synth_rings(int lmax, int spin, int ncomp, const double* alm, int 
 int nusht_cpu_adjoint_rings(int lmax, int spin, int ncomp, const double* G, int nring, const double* theta,
                             double* alm, int nthreads);
 
-/* ES-kernel interpolation out[p] = sum_{i,j} grid[i][j] psi(theta_p - 2 pi (row_lo + i) / nf1)
- * psi(phi_p - 2 pi j / nf2) from a window of nrow rows starting at torus row row_lo of an nf1 x nf2 periodic
- * grid (complex128, row-major (nrow, nf2)); psi(t) = exp(beta (sqrt(1 - z^2) - 1)), z = t nf / (pi w), width w
- * cells.  nrow == nf1 (with row_lo == 0) is the whole torus and wraps; a smaller window drops rows outside it.
- * Replaces ducc0.nufft.u2nu's spreading stage (CPU_nufft_transformer.py:346-351). */
-int nusht_cpu_interp(int nf1, int nf2, int row_lo, int nrow, int w, double beta, const double* grid, int64_t npts,
-                     const double* theta, const double* phi, double* out, int nthreads);
+/* A window of rows of an nf1 x nf2 periodic fine grid and its ES kernel.  The window holds nrow rows from
+ * torus row row_lo (complex128, row-major (nrow, nf2)); nrow == nf1 (with row_lo == 0) is the whole torus and
+ * wraps, a smaller window drops rows outside it.  Kernel psi(z) = exp(beta (sqrt(1 - z^2) - 1)) over w cells,
+ * z = 2 (distance in cells) / w, evaluated from per-cell polynomials fitted to max error ktol (a cell that
+ * no polynomial of degree <= 24 fits that well, at most the two outer ones in practice, is evaluated exactly). */
+typedef struct nusht_cpu_grid {
+    int nf1, nf2;      /* torus size (theta, phi) */
+    int row_lo, nrow;  /* the window of rows held */
+    int w;             /* kernel width in cells, 2..32 */
+    double beta;       /* kernel shape */
+    double ktol;       /* max error of the piecewise-polynomial kernel */
+} nusht_cpu_grid;
 
-/* Exact adjoint of nusht_cpu_interp: grid += sum_p in[p] psi psi (grid must be zeroed by the caller).
- * Replaces ducc0.nufft.nu2u's spreading stage (CPU_nufft_transformer.py:336-344). */
-int nusht_cpu_spread(int nf1, int nf2, int row_lo, int nrow, int w, double beta, const double* in, int64_t npts,
-                     const double* theta, const double* phi, double* grid, int nthreads);
+/* Degree and max error of the kernel polynomials for (w, beta, ktol) (no counterpart: diagnostics); cells
+ * evaluated exactly are not counted in max_err. */
+int nusht_cpu_es_kernel_info(int w, double beta, double ktol, int* degree, double* max_err);
+
+/* ES-kernel interpolation out[p] = e^{i (mshift phi_p + spin gamma_p)} sum_{i,j} grid[i][j]
+ * psi(theta_p - 2 pi (row_lo + i) / nf1) psi(phi_p - 2 pi j / nf2) (gamma may be NULL: no spin term).
+ * Replaces ducc0.nufft.u2nu's interpolation stage and the rotation ducc0.misc.lensing_rotate
+ * (CPU_nufft_transformer.py:346-368). */
+int nusht_cpu_interp(const nusht_cpu_grid* g, const double* grid, int64_t npts, const double* theta,
+                     const double* phi, int mshift, const double* gamma, int spin, double* out, int nthreads);
+
+/* Exact adjoint of nusht_cpu_interp: grid += sum_p e^{-i (mshift phi_p + spin gamma_p)} in[p] psi psi (grid
+ * zeroed by the caller).  Replaces ducc0.nufft.nu2u's spreading stage (CPU_nufft_transformer.py:336-344). */
+int nusht_cpu_spread(const nusht_cpu_grid* g, const double* in, int64_t npts, const double* theta,
+                     const double* phi, int mshift, const double* gamma, int spin, double* grid, int nthreads);
 
 /* Deflected angles (tp, pp) of points (theta, phi) moved along the great circle by the deflection
  * (dt, dp) = (d_theta, d_phi / sin theta components), and the rotation angle gamma when gamma != NULL.

@@ -1,13 +1,16 @@
 """The two backends agree: get_transformer('GPU') (HIP library) against get_transformer('CPU') (host library).
 
-Both are within epsilon of the exact transform, so they agree within 2 epsilon (relative max error) for
-epsilon = 1e-10; the pointing (same closed form, FP64 on both sides) within 1e-12.
+Each is checked against direct FP64 summation at the deflected points (oracle/sht.py) within 5 epsilon
+(relative max error; the ES kernel parameters target epsilon and reach 1-2 epsilon on these cases, the
+max over points is a small multiple of the norm-wise error), and they agree with each other within 8 epsilon,
+epsilon = 1e-10; the adjoints agree within 8 epsilon; the pointing (the same closed form in FP64 on both
+sides) within 1e-12.
 """
 import numpy as np
 import pytest
 import torch
 
-from oracle import almutil
+from oracle import almutil, sht as osht
 
 pytestmark = pytest.mark.gpu
 
@@ -35,10 +38,17 @@ def test_gpu_and_cpu_backends_agree(geom, spin):
     torch.cuda.synchronize()
     oc = C.gclm2lenmap(g, dlm, L, L, spin, eps, 8, verbose=0)
     assert _relmax(G.ptg.cpu().numpy(), C.deflectionlib.ptg()) < 1e-12
-    assert _relmax(og.cpu().numpy(), oc) < 2 * eps
+    tp, pp = C.deflectionlib.ptg()
+    ex = osht.synth_direct(g, L, spin, tp, pp)
+    if spin:
+        ex = ex * np.exp(1j * spin * C.deflectionlib.gamma())
+        ex = np.array([ex.real, ex.imag])
+    assert _relmax(oc, ex) < 5 * eps
+    assert _relmax(og.cpu().numpy(), ex) < 5 * eps
+    assert _relmax(og.cpu().numpy(), oc) < 8 * eps
     x = rng.standard_normal(oc.shape)
     ag = G.lenmap2gclm(torch.as_tensor(x, device="cuda"), torch.as_tensor(dlm_s, device="cuda"), None, L, L,
                        spin=spin)
     torch.cuda.synchronize()
     ac = C.lenmap2gclm(x, dlm, None, spin, L, L, 8, epsilon=eps, verbose=0)
-    assert _relmax(np.atleast_2d(ag.cpu().numpy()), ac) < 2 * eps
+    assert _relmax(np.atleast_2d(ag.cpu().numpy()), ac) < 8 * eps
