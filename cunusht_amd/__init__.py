@@ -19,19 +19,22 @@ def get_transformer(backend: str, solver: str = "default", mode: str = "nuFFT"):
     """Returns the transformer class for `backend` (cunusht/cunusht.py:63-82), mode 'nuFFT'.
 
     backend 'GPU': solver 'default' / 'cufinufft' / 'amd' -> GPU_nusht_transformer (libnusht_amd.so).
-    backend 'CPU': solver 'default' / 'duccnufft' / 'amd' -> CPU_nusht_transformer (libnusht_cpu.so), the
-    algorithm and signatures of the reference's CPU_DUCCnufft_transformer.  The reference's other CPU solvers
-    (lenspyx, finufft) wrap third-party packages and are not provided.
+    backend 'CPU' (libnusht_cpu.so): solver 'duccnufft' / 'amd' -> CPU_nusht_transformer, the algorithm and
+    signatures of the reference's CPU_DUCCnufft_transformer; solver 'default' / 'lenspyx' ->
+    CPU_nusht_lenspyx_transformer, the same operators with the signatures of the reference's default CPU solver
+    CPU_Lenspyx_transformer.  The reference's 'finufft' CPU solver (broken there) is not provided.
     """
     if backend not in ("GPU", "CPU"):
         raise AssertionError(f"{backend} backend not found")
     if mode not in ("nuFFT",):
         raise AssertionError(f"{mode} mode not found")
     if backend == "CPU":
-        if solver not in ("default", "duccnufft", "amd"):
-            raise AssertionError(f"Solver not found: {solver}")
-        from .cpu_transformer import CPU_nusht_transformer
-        return CPU_nusht_transformer
+        from .cpu_transformer import CPU_nusht_lenspyx_transformer, CPU_nusht_transformer
+        if solver in ("default", "lenspyx"):
+            return CPU_nusht_lenspyx_transformer
+        if solver in ("duccnufft", "amd"):
+            return CPU_nusht_transformer
+        raise AssertionError(f"Solver not found: {solver}")
     if solver not in ("default", "cufinufft", "amd"):
         raise AssertionError(f"Solver not found: {solver}")
     from .transformer import GPU_nusht_transformer

@@ -775,7 +775,11 @@ class deflection:
         return out.astype(np.float32) if single_prec else out.copy()
 
     def set_ptg(self, ptg_npix2):
-        self._ptg = np.ascontiguousarray(np.asarray(ptg_npix2, dtype=np.float64).T)
+        """A given pointing (npix, 2); the rotation angle is kept when it is the pointing computed last."""
+        p = np.ascontiguousarray(np.asarray(ptg_npix2, dtype=np.float64).T)
+        if self._ptg is not None and p.shape == self._ptg.shape and np.array_equal(p, self._ptg):
+            return
+        self._ptg = p
         self._gamma = None
         self._key = None
 
@@ -784,3 +788,83 @@ class deflection:
 
     def gamma(self):
         return self._gamma
+
+
+class CPU_nusht_lenspyx_transformer(CPU_nusht_transformer):
+    """The same CPU operators with the signatures of the reference's default CPU solver, CPU_Lenspyx_transformer
+    (cunusht/deflection/CPU_nufft_transformer.py:560-762), which wraps lenspyx / ducc0 synthesis_general.
+
+    Differences, on purpose: the transforms are this package's CC + ES-nuFFT route (within epsilon of the exact
+    synthesis_general the reference calls); gclm2lenmap honours `polrot` (the reference passes it but never
+    rotates) and returns the maps of the DUCC-signature class (real (npix,) for spin 0, (2, npix) for spin > 0);
+    `backwards` is accepted and, as in the reference, unused."""
+
+    def __init__(self, geominfo_deflection, dglm=None, mmax_dlm=None, nthreads=None, verbose=None, epsilon=None,
+                 single_prec=None):
+        super().__init__(geominfo_deflection, epsilon=epsilon)
+        self.dglm = None if dglm is None else np.asarray(dglm)
+        self.mmax_dlm = mmax_dlm
+        self.nthreads = nthreads
+        self.verbosity = 0 if verbose is None else verbose
+        if single_prec is not None:
+            self.single_prec = bool(single_prec)
+
+    def _dlm(self, dlm, dclm=None):
+        d = self.dglm if dlm is None else np.asarray(dlm)
+        if d is None:
+            raise ValueError("no deflection: pass dlm (or dglm at construction)")
+        if dclm is not None:
+            d = np.array([np.asarray(d).reshape(-1), np.asarray(dclm).reshape(-1)])
+        return d
+
+    def synthesis(self, alm, spin, out=None, lmax=None, mmax=None, nthreads=None):
+        """(ncomp, npix) maps on the pointing geometry (:629-635)."""
+        lmax = self.lmax if lmax is None else lmax
+        return self.deflectionlib.synthesis(alm, spin, lmax, lmax if mmax is None else mmax, nthreads, map=out,
+                                            mode=_mode(alm, spin))
+
+    def synthesis_general(self, lmax, mmax, alm, loc, spin, epsilon, nthreads, mode, verbose):
+        """Real maps (ncomp, npts) at loc (npts, 2) (:643-647, ducc0 synthesis_general's layout)."""
+        u = CPU_nusht_transformer.synthesis_general(self, None, lmax, mmax, alm, loc, spin, epsilon, nthreads, mode,
+                                                    verbose)
+        return u.real[None, :] if spin == 0 else np.array([u.real, u.imag])
+
+    def adjoint_synthesis_general(self, lmax, mmax, pointmap, loc, spin, epsilon, mode, nthreads, alm, verbose):
+        """Exact adjoint of synthesis_general: real maps (ncomp, npts) -> alm (:649-653)."""
+        m = np.atleast_2d(np.asarray(pointmap))
+        v = m[0] if spin == 0 else (m[0] + 1j * m[1] if m.shape[0] == 2 else m[0])
+        return CPU_nusht_transformer.adjoint_synthesis_general(self, lmax, mmax, v, loc, spin, epsilon, nthreads,
+                                                               mode, alm, verbose)
+
+    def dlm2pointing(self, dlm=None, mmax_dlm=None, single_prec=None, nthreads=None, epsilon=1e-14, verbose=1):
+        """Deflected (theta, phi) as (npix, 2) (:655-663)."""
+        d = self._dlm(dlm)
+        lmax = Alm.getlmax(np.atleast_2d(d)[0].size, mmax_dlm)
+        ptg = self.deflectionlib.dlm2pointing(np.atleast_2d(d)[0], lmax, bool(single_prec), nthreads)
+        return np.ascontiguousarray(ptg.T)
+
+    def gclm2lenmap(self, gclm, lmax, mmax, spin, nthreads, dlm=None, epsilon=None, backwards=False, polrot=True,
+                    ptg=None, dclm=None, lenmap=None, verbose=None, execmode="normal"):
+        """Lensed map(s) (:664-696)."""
+        eps = self.epsilon if epsilon is None else epsilon
+        assert eps is not None
+        return CPU_nusht_transformer.gclm2lenmap(self, gclm, None if ptg is not None else self._dlm(dlm, dclm),
+                                                 lmax, mmax, spin, eps, nthreads, polrot=polrot, ptg=ptg,
+                                                 lenmap=lenmap, verbose=self.verbosity if verbose is None else verbose,
+                                                 execmode=execmode)
+
+    def lenmap2gclm(self, lenmap, dlm, spin, lmax, mmax, nthreads, epsilon=None, gclm_out=None, ptg=None,
+                    verbose=None, execmode="normal", polrot=True):
+        """Exact adjoint of gclm2lenmap (:698-731)."""
+        eps = self.epsilon if epsilon is None else epsilon
+        assert eps is not None
+        self.epsilon = eps
+        return CPU_nusht_transformer.lenmap2gclm(self, lenmap, None if ptg is not None else self._dlm(dlm),
+                                                 gclm_out, spin, lmax, mmax, nthreads, epsilon=eps, ptg=ptg,
+                                                 verbose=self.verbosity if verbose is None else verbose,
+                                                 execmode=execmode, polrot=polrot)
+
+    def gclm2lenpixs(self, gclm, mmax, spin, pixs, polrot=True, ptg=None):
+        """As the DUCC-signature method; ptg is (npix, 2) here (:734-762)."""
+        return CPU_nusht_transformer.gclm2lenpixs(self, gclm, mmax, spin, pixs, polrot=polrot,
+                                                  ptg=None if ptg is None else np.asarray(ptg).T)

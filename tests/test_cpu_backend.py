@@ -43,9 +43,11 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_get_transformer_cpu():
+    """The reference's factory: default / lenspyx and duccnufft solvers (cunusht/cunusht.py:18-27,63-73)."""
     import cunusht_amd
-    from cunusht_amd.cpu_transformer import CPU_nusht_transformer
-    assert cunusht_amd.get_transformer("CPU") is CPU_nusht_transformer
+    from cunusht_amd.cpu_transformer import CPU_nusht_lenspyx_transformer, CPU_nusht_transformer
+    assert cunusht_amd.get_transformer("CPU") is CPU_nusht_lenspyx_transformer
+    assert cunusht_amd.get_transformer("CPU", solver="lenspyx") is CPU_nusht_lenspyx_transformer
     assert cunusht_amd.get_transformer("CPU", solver="duccnufft") is CPU_nusht_transformer
     with pytest.raises(AssertionError):
         cunusht_amd.get_transformer("CPU", solver="finufft")
@@ -110,7 +112,8 @@ def _case(L, spin, rng, scale=1e-5):
 
 def _transformer(L, geom=None, eps=1e-12):
     import cunusht_amd
-    return cunusht_amd.get_transformer("CPU")(geom or ("gl", {"lmax": L, "nphi": 2 * L + 1}), epsilon=eps)
+    return cunusht_amd.get_transformer("CPU", solver="duccnufft")(geom or ("gl", {"lmax": L, "nphi": 2 * L + 1}),
+                                                                  epsilon=eps)
 
 
 @pytest.mark.parametrize("L,spin", [(24, 0), (24, 2), (40, 1), (33, 3), (64, 2)])
@@ -289,3 +292,35 @@ def test_error_paths():
         T2.gclm2lenmap(g, None, L, L, 2, 1e-12, 1, ptg=bad, polrot=False, verbose=0)
     with pytest.raises(ValueError):
         T.gclm2lenmap(g, dlm, L, L, 2, 1.5, 1, verbose=0)
+
+
+def test_lenspyx_signatures():
+    """The default CPU solver's signatures (CPU_nufft_transformer.py:560-762) give the same operators."""
+    import cunusht_amd
+    L, eps = 20, 1e-12
+    rng = np.random.default_rng(16)
+    _, dlm, g, _ = _case(L, 2, rng)
+    geom = ("gl", {"lmax": L, "nphi": 2 * L + 1})
+    T = cunusht_amd.get_transformer("CPU")(geom, dglm=dlm, mmax_dlm=L, nthreads=2, verbose=0, epsilon=eps,
+                                           single_prec=False)
+    D = _transformer(L)
+    ref = D.gclm2lenmap(g, dlm, L, L, 2, eps, 2, verbose=0)
+    out = T.gclm2lenmap(g, L, L, 2, 2)                       # deflection from the constructor
+    assert np.array_equal(out, ref)
+    ptg = T.dlm2pointing(dlm, L, False, 2)
+    assert ptg.shape == (T.geom.npix(), 2)
+    assert np.array_equal(T.gclm2lenmap(g, L, L, 2, 2, ptg=ptg), ref)   # rotation angle kept for its own ptg
+    x = rng.standard_normal(ref.shape)
+    a = T.lenmap2gclm(x, dlm, 2, L, L, 2, epsilon=eps, gclm_out=np.zeros((2, almutil.nalm(L)), np.complex128))
+    assert np.array_equal(a, D.lenmap2gclm(x, dlm, None, 2, L, L, 2, epsilon=eps, verbose=0))
+    # synthesis_general: real (ncomp, npts) maps; its adjoint takes them back
+    loc = ptg
+    m = T.synthesis_general(L, L, g, loc, 2, eps, 2, "STANDARD", 0)
+    assert m.shape == (2, loc.shape[0])
+    y = rng.standard_normal(m.shape)
+    b = T.adjoint_synthesis_general(L, L, y, loc, 2, eps, "STANDARD", 2, None, 0)
+    lhs = float(np.sum(m * y))
+    assert abs(lhs - sum(almutil.myalmdot(g[c], b[c], L) for c in range(2))) < 1e-12 * abs(lhs)
+    q, u = T.gclm2lenpixs(g, L, 2, np.array([3, 50, 400]))
+    assert _relmax(np.array([q, u]), ref[:, [3, 50, 400]]) < 1e-11
+    assert T.synthesis(g, 2).shape == (2, T.geom.npix())

@@ -32,7 +32,7 @@ def test_gpu_and_cpu_backends_agree(geom, spin):
     g = almutil.random_alm(L, spin, nc, rng)
     print(f"case geom={geom[0]} spin={spin}", flush=True)
     G = cunusht_amd.get_transformer("GPU")(geom, nuFFTtype=2, epsilon=eps, spin=spin)
-    C = cunusht_amd.get_transformer("CPU")(geom, epsilon=eps)
+    C = cunusht_amd.get_transformer("CPU", solver="duccnufft")(geom, epsilon=eps)
     og = G.gclm2lenmap(torch.as_tensor(g, device="cuda"), L, L, dlm_scaled=torch.as_tensor(dlm_s, device="cuda"),
                        spin=spin)
     torch.cuda.synchronize()
