@@ -141,6 +141,38 @@ def cpu_baseline(lmax_sample: int, lmax: int, eps: float):
                       f"scaled x{f_leg:.1f} by N_lm * rings, rest {t_rest:.1f} s scaled x{f_rest:.1f} by pixels)"}
 
 
+def cpu_backend_rate(lmax_sample: int, lmax: int, eps: float):
+    """This package's CPU backend (get_transformer('CPU', solver='duccnufft'), libnusht_cpu.so) on the same
+    bounded sample as the oracle baseline, scaled the same way: Legendre stages by N_lm * rings, the rest by
+    pixels.  Informational: the product's host path beside the GPU, not the baseline."""
+    import cunusht_amd
+    from cunusht_amd.utils import almxfl
+    rng = np.random.default_rng(2024)
+    gclm, dlm_scaled = synth_inputs(lmax_sample, rng)
+    L = lmax_sample
+    ll = np.arange(L + 1, dtype=np.float64)
+    dlm = almxfl(dlm_scaled, np.sqrt(ll * (ll + 1)))
+    T = cunusht_amd.get_transformer("CPU", solver="duccnufft")(("gl", {"lmax": L, "nphi": 2 * L + 1}), epsilon=eps)
+    from cunusht_amd.cpu_transformer import synth_rings
+    nth = len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    T.gclm2lenmap(gclm, dlm, L, L, 2, eps, nth, verbose=0)  # pointing included, as in the oracle sample
+    t_total = time.perf_counter() - t0
+    ta = time.perf_counter()  # the spin-1 Legendre stage of the pointing, timed on its own
+    synth_rings(dlm[None, :], L, 1, T.geom.theta, nth)
+    t_leg1 = time.perf_counter() - ta
+    t_leg = T.timer.keys.get("synthesis", 0.0) + t_leg1
+    t_rest = max(t_total - t_leg, 0.0)
+    nlm_s, nlm = (L + 1) * (L + 2) // 2, (lmax + 1) * (lmax + 2) // 2
+    nt_s, nt = geometry_ntheta_ref(L), geometry_ntheta_ref(lmax)
+    f_leg = (nlm * (lmax + 1 + nt)) / (nlm_s * (L + 1 + nt_s))
+    f_rest = ((lmax + 1) * (2 * lmax + 1)) / ((L + 1) * (2 * L + 1))
+    return {"value": 1.0 / (t_leg * f_leg + t_rest * f_rest), "unit": "lensed maps/s", "cores": nth,
+            "sample": f"CPU backend gclm2lenmap spin-2 lmax={L} eps={eps:g} on the GL {L + 1}x{2 * L + 1} grid, "
+                      f"{t_total:.1f} s (Legendre, spin-1 pointing + spin-2 CC, {t_leg:.1f} s scaled x{f_leg:.1f}, "
+                      f"rest {t_rest:.1f} s scaled x{f_rest:.1f})"}
+
+
 def serve_headline(args, L, eps, gclms, dlms, rng):
     """Replay spin-2 gclm2lenmap requests through the continuous-batching server: Poisson arrivals at
     args.serve_rate, four deflections each shared by a quarter of the requests (same-deflection requests
@@ -404,10 +436,12 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    cpu = None
+    cpu = cpu_be = None
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         cpu = cpu_baseline(args.cpu_lmax, L, eps)
+        print("[bench] cpu backend ...", file=sys.stderr, flush=True)
+        cpu_be = cpu_backend_rate(args.cpu_lmax, L, eps)
 
     res = {
         "metric": "spin-2 lensed maps/s at lmax 4095, eps 1e-7 (served vs plain); p50/p99 ms",
@@ -443,6 +477,7 @@ def main():
         "stages_ms_per_step": {k: v[0] / args.steps for k, v in sorted(stages_batch.items())},
         "stages_plain_ms_per_call": {k: v[0] / args.steps for k, v in sorted(stages_plain.items())},
         "cpu_baseline": cpu,
+        "cpu_backend": cpu_be,
     }
     print(json.dumps(res))
 
