@@ -125,6 +125,7 @@ int nusht_pointplan_create(int lmax, int spin, int64_t npts, const double* theta
   return guard([&] {
     NUSHT_REQUIRE(theta && out, kErrArg, "null argument");
     NUSHT_REQUIRE(lmax >= 0 && spin >= 0 && spin <= 3, kErrArg, "bad lmax or spin");
+    NUSHT_REQUIRE(lmax >= spin, kErrArg, "lmax must be >= spin (a spin-s field has no modes l < s)");
     std::vector<double> th(theta, theta + npts);
     auto p = std::make_unique<nusht_pointplan_s>();
     p->pp = std::make_unique<PointPlan>(lmax, spin, th, S(s));
@@ -152,6 +153,7 @@ int nusht_ringplan_create(const nusht_ring_desc* d, nusht_stream_t s, nusht_ring
     NUSHT_REQUIRE(d && out, kErrArg, "null argument");
     NUSHT_REQUIRE(d->nring >= 1 && d->theta && d->phi0 && d->nph && d->ofs, kErrArg, "bad ring description");
     NUSHT_REQUIRE(d->spin >= 0 && d->spin <= 2, kErrArg, "spin must be 0, 1 or 2");
+    NUSHT_REQUIRE(d->lmax >= d->spin, kErrArg, "lmax must be >= spin (a spin-s field has no modes l < s)");
     std::vector<double> th(d->theta, d->theta + d->nring), p0(d->phi0, d->phi0 + d->nring);
     std::vector<long> nph(d->nph, d->nph + d->nring), ofs(d->ofs, d->ofs + d->nring);
     std::vector<double> wt;
@@ -241,6 +243,7 @@ int nusht_plan_create(const nusht_plan_desc* d, nusht_stream_t s, nusht_plan_t* 
     NUSHT_REQUIRE(d && out, kErrArg, "null argument");
     NUSHT_REQUIRE(d->mmax == d->lmax, kErrArg, "mmax must equal lmax");
     NUSHT_REQUIRE(d->spin >= 0 && d->spin <= 2, kErrArg, "spin must be 0, 1 or 2");
+    NUSHT_REQUIRE(d->lmax >= d->spin, kErrArg, "lmax must be >= spin (a spin-s field has no modes l < s)");
     NUSHT_REQUIRE(d->ncomp == 1 || (d->ncomp == 2 && d->spin > 0), kErrArg, "ncomp must be 1, or 2 for spin > 0");
     auto p = std::make_unique<nusht_plan_s>();
     p->lmax = d->lmax;

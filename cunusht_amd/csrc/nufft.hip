@@ -729,20 +729,27 @@ NufftParams NufftParams::choose(int lmax, int spin, double eps, double sigma_hin
     nf = maxlen;
     while (nf % 2 || !fft_supported(nf)) --nf;
   }
-  p.nf = nf;
-  p.sigma = (double)nf / p.nm;
-  NUSHT_REQUIRE(p.sigma >= 1.2, kErrUnsupported, "nufft: fine grid too small for this lmax");
-  int w = (int)std::ceil(std::log(1.0 / eps) / (M_PI * std::sqrt(1.0 - 1.0 / p.sigma)));
+  const double sg = (double)nf / p.nm;
+  NUSHT_REQUIRE(sg >= 1.2, kErrUnsupported, "nufft: fine grid too small for this lmax");
+  int w = (int)std::ceil(std::log(1.0 / eps) / (M_PI * std::sqrt(1.0 - 1.0 / sg)));
   w = std::max(w, 2);
   NUSHT_REQUIRE(w <= kMaxW, kErrUnsupported, "nufft: kernel width exceeds the supported maximum");
   p.w = w;
-  p.beta = 0.97 * M_PI * w * (1.0 - 1.0 / (2.0 * p.sigma));
+  p.beta = 0.97 * M_PI * w * (1.0 - 1.0 / (2.0 * sg));
+  // Small lmax: widen the grid until the rows touched by theta in [0, pi] (about nf / 2 + w + 3) form a window
+  // that does not wrap around the torus, nf >= 2 w + 8 (the binning and the tile loads assume it).  The kernel
+  // keeps the width and shape of the unwidened grid: the wider grid only lowers the aliasing error, while a
+  // kernel narrowed to it would raise the truncation error.
+  while (nf < 2 * w + 8 && nf < maxlen) nf = fft_good_size(nf + 2);
+  while (nf % 2) nf = fft_good_size(nf + 1);
+  p.nf = nf;
+  p.sigma = (double)nf / p.nm;
   // fine rows touched by points with theta in [0, pi] (one cell of slack on both sides)
   int lo = -(w / 2) - 1;
   int hi = (int)std::ceil(0.5 * nf - 0.5 * w) + w;
+  NUSHT_REQUIRE(hi - lo + 1 < nf, kErrUnsupported, "nufft: fine grid too small for the kernel width");
   p.row_lo = lo;
-  p.nrow = std::min(nf, hi - lo + 1);
-  if (p.nrow == nf) p.row_lo = 0;
+  p.nrow = hi - lo + 1;
   return p;
 }
 

@@ -467,3 +467,64 @@ def test_dlm2A_zero_deflection_is_one():
     T = cunusht_amd.get_transformer("GPU")(("gl", {"lmax": L, "nphi": 33}))
     A = T.dlm2A(torch.zeros((L + 1) * (L + 2) // 2, dtype=torch.complex128, device="cuda"))
     assert torch.equal(A, torch.ones_like(A))
+
+
+# ------------------------------------------------------------------ small lmax (fine grid widened to a non-wrapping window)
+@pytest.mark.parametrize("L,spin", [(1, 0), (2, 0), (4, 0), (8, 0), (11, 0), (2, 2), (4, 2), (8, 2), (11, 2)])
+@pytest.mark.parametrize("eps", [1e-12, 1e-7, 1e-5])
+def test_small_lmax_vs_direct(L, spin, eps):
+    """nusht2d2 / nusht2d1 at small lmax, with points on both poles, against direct Y_lm summation (forward within
+    10 epsilon, 30 epsilon in FP32; the adjoint within 30 epsilon); the fine grid of these plans would be narrower
+    than the rows a point at a pole touches without widening."""
+    import cunusht_amd
+    rng = np.random.default_rng(500 + 10 * L + spin)
+    npts = 400
+    theta = np.concatenate([[0.0, np.pi, 1e-3, np.pi - 1e-3], np.arccos(rng.uniform(-1, 1, npts - 4))])
+    phi = np.concatenate([[0.3, 2 * np.pi - 1e-12, -1.0, 7.0], rng.uniform(0, 2 * np.pi, npts - 4)])
+    loc = np.stack([theta, phi], 1)
+    nc = 1 if spin == 0 else 2
+    a = almutil.random_alm(L, spin, nc, rng)
+    ain = a[0] if spin == 0 else a
+    print(f"case L={L} spin={spin} eps={eps}", flush=True)
+    T = cunusht_amd.get_transformer("GPU")(("gl", {"lmax": L}), nuFFTtype=2, epsilon=eps, spin=spin)
+    got = T.nusht2d2(L, L, _t(ain), _t(loc), spin=spin).cpu().numpy()
+    ref = osht.synth_direct(ain, L, spin, theta, phi)
+    if spin:
+        ref = np.array([ref.real, ref.imag])
+    tol = 10 * max(eps, 1e-13) if eps < 1e-6 else 30 * eps
+    assert np.all(np.isfinite(got))
+    assert _rel2(got, ref) < tol
+    x = rng.uniform(-0.5, 0.5, (nc, npts))
+    xin = x[0] if spin == 0 else x
+    ad = T.nusht2d1(L, L, _t(xin), _t(loc), spin=spin).cpu().numpy().reshape(nc, -1)
+    vals = xin if spin == 0 else (x[0] + 1j * x[1])
+    aref = osht.adjoint_direct(vals, L, spin, theta, phi).reshape(nc, -1)
+    # the adjoint sums 400 kernel-weighted values into a handful of coefficients: 30 epsilon
+    assert _rel2(ad, aref) < 30 * max(eps, 1e-12)
+
+
+@pytest.mark.parametrize("spin", [0, 2])
+def test_small_lmax_gclm2lenmap_vs_oracle(spin):
+    import cunusht_amd
+    L = 8
+    rng = np.random.default_rng(600 + spin)
+    ll = np.arange(L + 1)
+    dlm = almutil.synalm(np.where(ll >= 2, 1e-3 / (ll + 1.0) ** 2, 0.0), L, rng)
+    nc = 1 if spin == 0 else 2
+    g = almutil.random_alm(L, spin, nc, rng)
+    gin = g[0] if spin == 0 else g
+    print(f"case gclm2lenmap L={L} spin={spin}", flush=True)
+    T = cunusht_amd.get_transformer("GPU")(("gl", {"lmax": L, "nphi": 2 * L + 1}), nuFFTtype=2, epsilon=1e-12,
+                                           spin=spin)
+    out = T.gclm2lenmap(_t(gin), L, L, dlm_scaled=_t(dlm), spin=spin).cpu().numpy()
+    rings = ogeom.gl_geometry(L, nphi=2 * L + 1)
+    ref = pipeline.gclm2lenmap(g, dlm, L, spin, rings, 1e-12)
+    assert _relmax(out, ref) < 1e-9
+
+
+def test_lmax_below_spin_is_rejected():
+    """A spin-s field has no modes l < s: plans with lmax < spin are refused on the host."""
+    from cunusht_amd import _lib
+    from cunusht_amd.transformer import NuPlan
+    with pytest.raises(_lib.NuShtError, match="lmax must be >= spin"):
+        NuPlan(1, 2, 2, 1e-10, 10)
