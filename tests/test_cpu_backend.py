@@ -324,3 +324,23 @@ def test_lenspyx_signatures():
     q, u = T.gclm2lenpixs(g, L, 2, np.array([3, 50, 400]))
     assert _relmax(np.array([q, u]), ref[:, [3, 50, 400]]) < 1e-11
     assert T.synthesis(g, 2).shape == (2, T.geom.npix())
+
+
+@pytest.mark.parametrize("L,spin", [(1, 0), (2, 0), (2, 2), (3, 1)])
+def test_small_lmax(L, spin):
+    """Tiny lmax: the fine grid is the whole torus (rows wrap) and the GL geometry has a few rings."""
+    rng = np.random.default_rng(700 + 10 * L + spin)
+    _, dlm, g, nc = _case(L, spin, rng, scale=1e-3)
+    T = _transformer(L)
+    out = T.gclm2lenmap(g, dlm, L, L, spin, 1e-12, 2, verbose=0)
+    tp, pp = T.deflectionlib.ptg()
+    ex = osht.synth_direct(g, L, spin, tp, pp)
+    if spin:
+        ex = ex * np.exp(1j * spin * T.deflectionlib.gamma())
+        ex = np.array([ex.real, ex.imag])
+    assert _relmax(out, ex) < 1e-11
+    x = rng.standard_normal(out.shape)
+    a = T.lenmap2gclm(x, dlm, None, spin, L, L, 2, epsilon=1e-12, verbose=0)
+    lhs = float(np.sum(out * x))
+    rhs = sum(almutil.myalmdot(np.atleast_2d(g)[c], a[c], L) for c in range(nc))
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)

@@ -503,11 +503,11 @@ def test_small_lmax_vs_direct(L, spin, eps):
     assert _rel2(ad, aref) < 30 * max(eps, 1e-12)
 
 
+@pytest.mark.parametrize("L", [2, 8])
 @pytest.mark.parametrize("spin", [0, 2])
-def test_small_lmax_gclm2lenmap_vs_oracle(spin):
+def test_small_lmax_gclm2lenmap_vs_oracle(L, spin):
     import cunusht_amd
-    L = 8
-    rng = np.random.default_rng(600 + spin)
+    rng = np.random.default_rng(600 + spin + 10 * L)
     ll = np.arange(L + 1)
     dlm = almutil.synalm(np.where(ll >= 2, 1e-3 / (ll + 1.0) ** 2, 0.0), L, rng)
     nc = 1 if spin == 0 else 2
