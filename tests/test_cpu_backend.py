@@ -344,3 +344,22 @@ def test_small_lmax(L, spin):
     lhs = float(np.sum(out * x))
     rhs = sum(almutil.myalmdot(np.atleast_2d(g)[c], a[c], L) for c in range(nc))
     assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+
+
+@pytest.mark.parametrize("spin", [0, 2])
+def test_adjointness_reference_configuration(spin):
+    """The reference's dot-product test at its own settings (test/test_adjointOperator.py:247-277): lmax = mmax
+    = 1024, 20 000 points, epsilon 1e-8, rng(48), rtol 1e-9; spin 2 GRAD_ONLY; through the reference-signature
+    synthesis_general / adjoint_synthesis_general."""
+    lmax, npix, eps = 1024, 20000, 1e-8
+    rng = np.random.default_rng(48)
+    loc = np.stack([np.pi * rng.uniform(0, 1, npix), 2 * np.pi * rng.uniform(0, 1, npix)], 1)
+    a = almutil.random_alm(lmax, spin, 1, rng)
+    x = rng.uniform(-0.5, 0.5, npix) + (1j * rng.uniform(-0.5, 0.5, npix) if spin else 0)
+    T = _transformer(lmax, eps=eps)
+    mode = "GRAD_ONLY" if spin else "STANDARD"
+    y = T.synthesis_general(None, lmax, lmax, a, loc, spin, eps, 8, mode, 0)
+    ad = T.adjoint_synthesis_general(lmax, lmax, x, loc, spin, eps, 8, mode, None, 0)
+    v1 = almutil.myalmdot(a[0], ad[0], lmax)
+    v2 = float(np.real(np.vdot(y, x)))
+    np.testing.assert_allclose(v1, v2, rtol=1e-9)

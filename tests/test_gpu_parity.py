@@ -528,3 +528,25 @@ def test_lmax_below_spin_is_rejected():
     from cunusht_amd.transformer import NuPlan
     with pytest.raises(_lib.NuShtError, match="lmax must be >= spin"):
         NuPlan(1, 2, 2, 1e-10, 10)
+
+
+@pytest.mark.parametrize("spin", [0, 2])
+def test_adjointness_reference_configuration(spin):
+    """The reference's dot-product test at its own settings (test/test_adjointOperator.py:247-277): lmax = mmax
+    = 1024, 20 000 points (theta = pi U, phi = 2 pi U), epsilon 1e-8, rng(48), values U(-0.5, 0.5), rtol 1e-9;
+    spin 2 in the GRAD_ONLY form (one component)."""
+    import cunusht_amd
+    lmax, npix, eps = 1024, 20000, 1e-8
+    rng = np.random.default_rng(48)
+    loc = np.stack([np.pi * rng.uniform(0, 1, npix), 2 * np.pi * rng.uniform(0, 1, npix)], 1)
+    a = almutil.random_alm(lmax, spin, 1, rng)
+    x = rng.uniform(-0.5, 0.5, (1 if spin == 0 else 2, npix))
+    print(f"case reference adjointness spin={spin}", flush=True)
+    T = cunusht_amd.get_transformer("GPU")(("gl", {"lmax": lmax}), nuFFTtype=1, epsilon=eps, spin=spin)
+    y = T.nusht2d2(lmax, lmax, _t(a[0] if spin == 0 else a), _t(loc), spin=spin).cpu().numpy().reshape(x.shape)
+    alm_out = torch.empty((1, almutil.nalm(lmax)), dtype=torch.complex128, device="cuda")
+    ad = T.nusht2d1(lmax, lmax, _t(x[0] if spin == 0 else x), _t(loc), alm=alm_out,
+                    spin=spin).cpu().numpy().reshape(1, -1)
+    v1 = almutil.myalmdot(a[0], ad[0], lmax)
+    v2 = float(np.sum(x * y))
+    np.testing.assert_allclose(v1, v2, rtol=1e-9)
