@@ -179,6 +179,31 @@ def test_dlm2pointing_vs_oracle(L, nphi):
     assert np.max(np.abs(got[2] - g)) < 1e-10
 
 
+@pytest.mark.parametrize("lmax", [255, 511, 1023])
+def test_pointing_reference_configuration(lmax):
+    """test/test_pointingGPU.py:83-122 at its sizes: alm of amplitude 1e-6 (U(-1, 1) re / im), the GPU
+    gradient synthesis of alm sqrt(1/(l(l+1))) plus pointing against the CPU spin-1 synthesis of alm plus
+    deflected angles (the oracle); std(dtheta) <= 1e-15 and std(dphi) <= 1e-12 with dphi > pi masked."""
+    import cunusht_amd
+    rng = np.random.default_rng(lmax)
+    alm = 1e-6 * almutil.random_alm(lmax, 0, 1, rng)[0]
+    ll = np.arange(lmax + 1, dtype=np.float64)
+    with np.errstate(divide="ignore"):
+        fl = np.where(ll > 0, 1.0 / np.sqrt(ll * (ll + 1.0)), 0.0)
+    dlm_scaled = almutil.almxfl(alm, fl, lmax)
+    print(f"case pointing lmax={lmax}", flush=True)
+    T = cunusht_amd.get_transformer("GPU")(("gl", {"lmax": lmax}))
+    got = T.dlm2pointing(_t(dlm_scaled)).cpu().numpy()
+    rings = ogeom.gl_geometry(lmax, nphi=2 * lmax + 2)
+    assert rings.npix() == got.shape[1]
+    tp, pp, _, _ = pipeline.dlm2pointing(dlm_scaled, lmax, rings, with_gamma=False)
+    dth = got[0] - tp
+    dph = got[1] - pp
+    dph = dph[np.abs(dph) <= np.pi]
+    assert np.std(dth) <= 1e-15
+    assert np.std(dph) <= 1e-12
+
+
 @pytest.mark.parametrize("spin", [0, 1, 2])
 @pytest.mark.parametrize("mcut", [0, 40, 120])
 def test_ring_sht_polar_underflow(spin, mcut):
