@@ -279,6 +279,8 @@ def main():
     ap.add_argument("--eps", type=float, default=1e-7)
     ap.add_argument("--batch", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--split-batch", action="store_true",
+                    help="batched step as pointing, then nusht2d2_batch (no overlap of pointing and Legendre)")
     ap.add_argument("--cpu-lmax", type=int, default=2559)
     ap.add_argument("--serve-requests", type=int, default=32,
                     help="requests replayed through the serving loop (0 = skip); they share 4 deflections")
@@ -336,10 +338,20 @@ def main():
 
     def batched_step(i):
         d = dlms[i % nin]
-        _lib.call("nusht_ringplan_dlm2pointing", ring._h, P(d), P(ptg[0]), P(ptg[1]), P(ptg[2]), stream())
-        _lib.call("nusht_plan_set_pts_async", plan._h, P(ptg[0]), P(ptg[1]), npix, stream())
         plan.pts_key = None
         done = 0
+        if not args.split_batch:
+            # one call: pointing and binning on this stream, the Legendre synthesis of the first (up to 8)
+            # maps beside it on the plan's own stream
+            nb = min(8, B)
+            _lib.call("nusht_gclm2lenmap_batch", plan._h, ring._h, nb, P(gb[0]), nalm2, P(d), P(outb[0, 0]),
+                      P(outb[0, 1]), 2 * npix, 1, stream())
+            done = nb
+            if done < B:  # later chunks reuse the points; their rotation angle comes from the ring plan
+                _lib.call("nusht_ringplan_dlm2pointing", ring._h, P(d), P(ptg[0]), P(ptg[1]), P(ptg[2]), stream())
+        else:
+            _lib.call("nusht_ringplan_dlm2pointing", ring._h, P(d), P(ptg[0]), P(ptg[1]), P(ptg[2]), stream())
+            _lib.call("nusht_plan_set_pts_async", plan._h, P(ptg[0]), P(ptg[1]), npix, stream())
         while done < B:
             nb = min(8, B - done)
             _lib.call("nusht_nusht2d2_batch", plan._h, nb, P(gb[done]), nalm2, P(outb[done, 0]), P(outb[done, 1]),

@@ -104,6 +104,7 @@ SIGNATURES = {
     "nusht_gclm2lenmap": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _vp],
     "nusht_lenmap2gclm": [_vp, _vp, _vp, _vp, _vp, _vp, _i, _vp],
     "nusht_nusht2d2_batch": [_vp, _i, _vp, _i64, _vp, _vp, _i64, _vp, _vp],
+    "nusht_gclm2lenmap_batch": [_vp, _vp, _i, _vp, _i64, _vp, _vp, _vp, _i64, _i, _vp],
     "nusht_timing_enable": [_i],
     "nusht_timing_reset": [],
     "nusht_timing_count": [],

@@ -19,6 +19,28 @@
 
 using namespace nusht;
 
+// A second stream per plan for the forward lensing operator: the Legendre synthesis needs only the alm and
+// the pointing / binning only the deflection, so the first runs on `side` while the second runs on the
+// caller's stream, joined before the interpolation.  Created on first use, on the plan's device.
+struct ForkJoin {
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+  void ensure() {
+    if (side) return;
+    NUSHT_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    NUSHT_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    NUSHT_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+  }
+  ForkJoin() = default;
+  ForkJoin(const ForkJoin&) = delete;
+  ForkJoin& operator=(const ForkJoin&) = delete;
+  ~ForkJoin() {
+    if (join) (void)hipEventDestroy(join);
+    if (fork) (void)hipEventDestroy(fork);
+    if (side) (void)hipStreamDestroy(side);
+  }
+};
+
 struct nusht_plan_s {
   NufftParams np;
   int lmax = 0, spin = 0, ncomp = 1;
@@ -29,6 +51,7 @@ struct nusht_plan_s {
   int nmrows = 0;
   int max_maps = kMaxMaps;  // maps per batched synthesis the G buffer holds
   static constexpr int kMaxMaps = 8;
+  ForkJoin fj;
   size_t bytes() const { return tab.bytes() + nu->bytes() + coef.bytes() + G.bytes() + ptg.bytes(); }
 };
 
@@ -352,10 +375,29 @@ const double* lensing_points(nusht_plan_t p, nusht_ringplan_t ptg, const double*
   return rot ? ga : nullptr;
 }
 
+// nmaps lensed maps sharing one deflection: the Legendre synthesis of all maps on the plan's side stream
+// overlaps the pointing and binning on `st`; the interpolations follow on `st` once both are done.
+void op_gclm2lenmap_n(nusht_plan_t p, nusht_ringplan_t ptg, int nmaps, const double* gclm, long alm_stride,
+                      const double* dlm_scaled, double* out0, double* out1, long out_stride, int polrot,
+                      hipStream_t st) {
+  p->fj.ensure();
+  hipStream_t ss = p->fj.side;
+  NUSHT_HIP(hipEventRecord(p->fj.fork, st));
+  NUSHT_HIP(hipStreamWaitEvent(ss, p->fj.fork, 0));
+  const long gmap = (long)p->nmrows * p->np.ntheta;
+  legendre_synth(p->tab, nmaps, p->ncomp, reinterpret_cast<const double2*>(gclm), alm_stride / 2, nullptr,
+                 p->coef.as<double2>(), p->G.as<double2>(), p->np.ntheta, 1, gmap, ss);
+  NUSHT_HIP(hipEventRecord(p->fj.join, ss));
+  const double* ga = lensing_points(p, ptg, dlm_scaled, polrot, st);
+  NUSHT_HIP(hipStreamWaitEvent(st, p->fj.join, 0));
+  for (int b = 0; b < nmaps; ++b)
+    p->nu->type2(p->G.as<double2>() + b * gmap, p->np.ntheta, ga, out0 + b * out_stride,
+                 out1 ? out1 + b * out_stride : nullptr, st);
+}
+
 void op_gclm2lenmap(nusht_plan_t p, nusht_ringplan_t ptg, const double* gclm, const double* dlm_scaled,
                     double* lenmap0, double* lenmap1, int polrot, hipStream_t st) {
-  const double* ga = lensing_points(p, ptg, dlm_scaled, polrot, st);
-  synth_batch(p, 1, gclm, 0, lenmap0, lenmap1, 0, ga, st);
+  op_gclm2lenmap_n(p, ptg, 1, gclm, 0, dlm_scaled, lenmap0, lenmap1, 0, polrot, st);
 }
 
 void op_lenmap2gclm(nusht_plan_t p, nusht_ringplan_t ptg, const double* lenmap0, const double* lenmap1,
@@ -408,6 +450,20 @@ int nusht_gclm2lenmap(nusht_plan_t p, nusht_ringplan_t ptg, const double* gclm, 
     NUSHT_REQUIRE(gclm && dlm_scaled && lenmap0, kErrArg, "null array");
     NUSHT_REQUIRE(p->spin == 0 || lenmap1, kErrArg, "spin > 0 needs two output arrays");
     op_gclm2lenmap(p, ptg, gclm, dlm_scaled, lenmap0, lenmap1, polrot, S(s));
+  });
+}
+
+int nusht_gclm2lenmap_batch(nusht_plan_t p, nusht_ringplan_t ptg, int nmaps, const double* gclm,
+                            int64_t alm_stride, const double* dlm_scaled, double* lenmap0, double* lenmap1,
+                            int64_t out_stride, int polrot, nusht_stream_t s) {
+  return guard([&] {
+    check_plan(p);
+    check_lensing(p, ptg);
+    NUSHT_REQUIRE(nmaps >= 1 && nmaps <= p->max_maps, kErrArg,
+                  "nmaps must be in [1, " + std::to_string(p->max_maps) + "] (the plan's max_maps)");
+    NUSHT_REQUIRE(gclm && dlm_scaled && lenmap0, kErrArg, "null array");
+    NUSHT_REQUIRE(p->spin == 0 || lenmap1, kErrArg, "spin > 0 needs two output arrays");
+    op_gclm2lenmap_n(p, ptg, nmaps, gclm, alm_stride, dlm_scaled, lenmap0, lenmap1, out_stride, polrot, S(s));
   });
 }
 

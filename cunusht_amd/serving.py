@@ -10,7 +10,7 @@ the HBM budget:
   different configurations always do;
 * queued forward requests of one configuration that share a deflection (the same dlm_scaled tensor) are
   admitted together, up to `max_batch`, and run as one pointing + one batched synthesis
-  (nusht_nusht2d2_batch: one Legendre recursion for all of their maps);
+  (nusht_gclm2lenmap_batch: one Legendre recursion for all of their maps, beside the pointing);
 * plans stay resident; when a new plan would exceed the budget, idle plans of other configurations are
   released, least recently used first;
 * completion is observed through events (no device-wide synchronization), and every request records its
@@ -69,8 +69,6 @@ class _Slot:
         self.busy = False
         self.last_used = time.perf_counter()
         self.nbytes = self.plan.nbytes()
-        npix = self.plan.npix()
-        self.ptg = torch.empty((3, npix), dtype=torch.float64, device="cuda")
 
 
 class LensingServer:
@@ -228,12 +226,9 @@ class LensingServer:
                 for b, r in enumerate(reqs):
                     alm[b].copy_(r.data.reshape(p.ncomp, nalm))
                 out = torch.empty((nb, 2, npix), dtype=torch.float64, device="cuda")
-                rot = bool(r0.polrot and p.spin > 0)
-                _lib.call("nusht_ringplan_dlm2pointing", p.ring._h, _ptr(r0.dlm_scaled), _ptr(slot.ptg[0]),
-                          _ptr(slot.ptg[1]), _ptr(slot.ptg[2]) if rot else None, sp)
-                _lib.call("nusht_plan_set_pts_async", p.nuplan._h, _ptr(slot.ptg[0]), _ptr(slot.ptg[1]), npix, sp)
-                _lib.call("nusht_nusht2d2_batch", p.nuplan._h, nb, _ptr(alm), 2 * p.ncomp * nalm, _ptr(out[:, 0]),
-                          _ptr(out[:, 1]) if p.spin else None, 2 * npix, _ptr(slot.ptg[2]) if rot else None, sp)
+                _lib.call("nusht_gclm2lenmap_batch", p.nuplan._h, p.ring._h, nb, _ptr(alm), 2 * p.ncomp * nalm,
+                          _ptr(r0.dlm_scaled), _ptr(out[:, 0]), _ptr(out[:, 1]) if p.spin else None, 2 * npix,
+                          1 if r0.polrot else 0, sp)
                 for b, r in enumerate(reqs):
                     r.result = out[b, 0] if p.spin == 0 else out[b]
             ev = torch.cuda.Event()

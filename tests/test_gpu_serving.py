@@ -177,3 +177,31 @@ def test_server_admission_respects_budget():
     server = LensingServer(hbm_budget_bytes=1, plans_per_key=1, max_batch=1)
     stats = serve(server, reqs, np.zeros(len(reqs)))
     assert stats["requests"] == 6 and stats["plans"] == 1
+
+
+@pytest.mark.parametrize("spin,eps,nmaps", [(2, 1e-10, 5), (0, 1e-10, 3), (2, 1e-5, 2), (2, 1e-10, 1)])
+def test_gclm2lenmap_batch_equals_single_calls(spin, eps, nmaps):
+    """nusht_gclm2lenmap_batch (pointing once, Legendre synthesis forked beside it) == nmaps gclm2lenmap calls
+    with the same deflection; the call is repeated so that the second fork reuses the plan's side stream."""
+    from cunusht_amd import _lib
+    from cunusht_amd.batch import LensingPlan
+    L = 40
+    rng = np.random.default_rng(900 + spin + nmaps)
+    plan = LensingPlan(L, spin, eps, max_maps=8)
+    nc = 1 if spin == 0 else 2
+    dlm = _t(_deflection(L, rng))
+    g = _t(np.stack([almutil.random_alm(L, spin, nc, rng) for _ in range(nmaps)]))
+    npix = plan.npix()
+    out = torch.full((nmaps, 2, npix), np.nan, dtype=torch.float64, device="cuda")
+    s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    p = lambda t: ctypes.c_void_p(0 if t is None else t.data_ptr())
+    print(f"case gclm2lenmap_batch spin={spin} eps={eps} nmaps={nmaps}", flush=True)
+    for _ in range(2):
+        _lib.call("nusht_gclm2lenmap_batch", plan.nuplan._h, plan.ring._h, nmaps, p(g[0]), 2 * g[0].numel(), p(dlm),
+                  p(out[0, 0]), p(out[0, 1]) if spin else None, 2 * npix, 1, s)
+        torch.cuda.synchronize()
+        got = out.cpu().numpy()
+        for k in range(nmaps):
+            ref = _alone(plan, "gclm2lenmap", g[k], dlm)
+            tol = 1e-6 if eps >= 1e-6 else 1e-12
+            assert _relmax(got[k, 0] if spin == 0 else got[k], ref) < tol
