@@ -19,9 +19,9 @@
 
 using namespace nusht;
 
-// A second stream per plan for the forward lensing operator: the Legendre synthesis needs only the alm and
-// the pointing / binning only the deflection, so the first runs on `side` while the second runs on the
-// caller's stream, joined before the interpolation.  Created on first use, on the plan's device.
+// A second stream per plan for the batched forward lensing operator: the Legendre synthesis needs only the alm
+// and the pointing / binning only the deflection, so the first runs on `side` while the second runs on the
+// caller's stream, joined before the interpolation.  Created with the plan, on the plan's device.
 struct ForkJoin {
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
@@ -286,6 +286,7 @@ int nusht_plan_create(const nusht_plan_desc* d, nusht_stream_t s, nusht_plan_t* 
     p->G.alloc(sizeof(double2) * (size_t)p->nmrows * p->np.ntheta * p->max_maps);
     p->ptg.alloc(sizeof(double) * 3 * (size_t)p->npts_max);
     legendre_autotune(p->tab, p->ncomp, p->coef.as<double2>(), p->G.as<double2>(), p->np.ntheta, 1, S(s));
+    p->fj.ensure();  // at creation, so that no stream is created under a caller's graph capture
     NUSHT_HIP(hipStreamSynchronize(S(s)));
     *out = p.release();
   });
@@ -375,11 +376,27 @@ const double* lensing_points(nusht_plan_t p, nusht_ringplan_t ptg, const double*
   return rot ? ga : nullptr;
 }
 
+void op_gclm2lenmap(nusht_plan_t p, nusht_ringplan_t ptg, const double* gclm, const double* dlm_scaled,
+                    double* lenmap0, double* lenmap1, int polrot, hipStream_t st) {
+  const double* ga = lensing_points(p, ptg, dlm_scaled, polrot, st);
+  synth_batch(p, 1, gclm, 0, lenmap0, lenmap1, 0, ga, st);
+}
+
 // nmaps lensed maps sharing one deflection: the Legendre synthesis of all maps on the plan's side stream
-// overlaps the pointing and binning on `st`; the interpolations follow on `st` once both are done.
+// overlaps the pointing and binning on `st`; the interpolations follow on `st` once both are done.  Under a
+// graph capture of `st` the steps run in order on `st` instead: a fork from a stream that had itself joined a
+// capture through an event (the batch executor's streams) made hipStreamEndCapture crash, so no fork is
+// recorded into a graph.
 void op_gclm2lenmap_n(nusht_plan_t p, nusht_ringplan_t ptg, int nmaps, const double* gclm, long alm_stride,
                       const double* dlm_scaled, double* out0, double* out1, long out_stride, int polrot,
                       hipStream_t st) {
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  NUSHT_HIP(hipStreamIsCapturing(st, &cap));
+  if (cap != hipStreamCaptureStatusNone) {
+    const double* ga = lensing_points(p, ptg, dlm_scaled, polrot, st);
+    synth_batch(p, nmaps, gclm, alm_stride, out0, out1, out_stride, ga, st);
+    return;
+  }
   p->fj.ensure();
   hipStream_t ss = p->fj.side;
   NUSHT_HIP(hipEventRecord(p->fj.fork, st));
@@ -393,11 +410,6 @@ void op_gclm2lenmap_n(nusht_plan_t p, nusht_ringplan_t ptg, int nmaps, const dou
   for (int b = 0; b < nmaps; ++b)
     p->nu->type2(p->G.as<double2>() + b * gmap, p->np.ntheta, ga, out0 + b * out_stride,
                  out1 ? out1 + b * out_stride : nullptr, st);
-}
-
-void op_gclm2lenmap(nusht_plan_t p, nusht_ringplan_t ptg, const double* gclm, const double* dlm_scaled,
-                    double* lenmap0, double* lenmap1, int polrot, hipStream_t st) {
-  op_gclm2lenmap_n(p, ptg, 1, gclm, 0, dlm_scaled, lenmap0, lenmap1, 0, polrot, st);
 }
 
 void op_lenmap2gclm(nusht_plan_t p, nusht_ringplan_t ptg, const double* lenmap0, const double* lenmap1,

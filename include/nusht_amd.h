@@ -250,8 +250,8 @@ int nusht_nusht2d2_batch(nusht_plan_t p, int nmaps, const double* alm, int64_t a
 /* Batched lensing: nmaps gclm sets (stride alm_stride doubles) lensed by one deflection, i.e. nmaps
  * gclm2lenmap calls sharing dlm_scaled (GPU_nufft_transformer.py:486-578 per map), with the pointing and binning
  * done once.  Outputs at lenmap0 + k * out_stride (and lenmap1).  The Legendre synthesis runs on a stream of
- * the plan's own, forked from and joined back into `s`, beside the pointing on `s` (nusht_gclm2lenmap does the
- * same for one map); calls on one plan must not overlap. */
+ * the plan's own, forked from and joined back into `s`, beside the pointing on `s`; while `s` is being
+ * captured into a graph the steps run in order on `s` instead.  Calls on one plan must not overlap. */
 int nusht_gclm2lenmap_batch(nusht_plan_t p, nusht_ringplan_t ptg, int nmaps, const double* gclm,
                             int64_t alm_stride, const double* dlm_scaled, double* lenmap0, double* lenmap1,
                             int64_t out_stride, int polrot, nusht_stream_t s);
