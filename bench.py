@@ -398,6 +398,24 @@ def main():
     rate_plain = args.steps / t_plain
     print(f"[bench] batched: {rate_batch:.2f} maps/s", file=sys.stderr, flush=True)
 
+    # accuracy of both paths against exact FP64 synthesis at sampled pixels of the deflected grid (the
+    # library's point plan: Legendre recursion on the pixels' own colatitudes and a direct sum over m, no nuFFT)
+    batched_step(0)  # deflection dlms[0]; map 0 of the batch is gclms[0]
+    plain_step(0)    # the same map and deflection alone; leaves T's pointing and rotation angle in place
+    torch.cuda.synchronize()
+    pix = np.sort(np.random.default_rng(77).choice(npix, 256, replace=False))
+    q, u = T.gclm2lenpixs(gclms[0], L, 2, pix)
+    ref = torch.stack([q, u]).cpu().numpy()
+
+    def _err(m):
+        d = m[:, pix].cpu().numpy() - ref
+        return {"max_rel": float(np.max(np.abs(d)) / np.max(np.abs(ref))),
+                "rms_rel": float(np.sqrt(np.mean(d * d)) / np.sqrt(np.mean(ref * ref)))}
+
+    accuracy = {"reference": "exact FP64 synthesis at 256 random pixels of the deflected grid (point plan)",
+                "batched": _err(outb[0]), "plain": _err(out)}
+    print(f"[bench] accuracy: {accuracy}", file=sys.stderr, flush=True)
+
     # roofline of the dominant stage of the headline (batched) path
     dom = max(stages_batch.items(), key=lambda kv: kv[1][0])
     name, (tot_ms, calls) = dom
@@ -484,6 +502,7 @@ def main():
         "served": served,
         "batched_over_plain": rate_batch / rate_plain,
         "served_over_plain": (served["maps_per_s"] / rate_plain) if served else None,
+        "accuracy": accuracy,
         "roofline": roof,
         "nufft_interp_roofline": nufft_roof,
         "stages_ms_per_step": {k: v[0] / args.steps for k, v in sorted(stages_batch.items())},
