@@ -14,6 +14,8 @@ roofline of the dominant kernel (live hipEvent timing inside the library) and an
 baseline are reported beside it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--lmax L] [--eps E] [--batch B] [--no-cpu]
+       python bench.py --uneven-batch 64      (64 independent transforms of uneven lmax / spin)
+       python bench.py --serve-mix 48         (served mix of lmax 255..4095, spin 0/2, three eps, both directions)
 """
 from __future__ import annotations
 
@@ -265,6 +267,105 @@ def uneven_batch(njobs: int, steps: int, warmup: int):
             "mix": {f"{L}/{s}": int(np.sum((lmaxs == L) & (spins == s))) for (L, s) in sorted(plans)}}
 
 
+def serve_mix(nreq: int, rate: float):
+    """The serving configuration: Poisson arrivals at `rate`, each request forward (gclm2lenmap) or adjoint
+    (lenmap2gclm), lmax ~ U{255, 511, 1023, 2047, 4095}, spin ~ U{0, 2}, eps ~ U{1e-5, 1e-7, 1e-10}; seeded
+    gclm (C_l ~ (l+1)^-2) or N(0, 1) maps, one 2.5 arcmin deflection per lmax (shared, so forward requests of one
+    configuration can batch).  Served through LensingServer (one resident plan per configuration, admission
+    under a 200 GiB budget, batches of up to 8) twice: cold (plans built on demand) and warm (plans resident);
+    beside it the plain path: the same requests one call at a time on resident plans.  Latencies: arrival to
+    completion (served), call to completion (plain)."""
+    import ctypes
+
+    import torch
+    from cunusht_amd.batch import LensingPlan
+    from cunusht_amd.serving import LensingServer, Request, poisson_arrivals, serve
+    from cunusht_amd.utils import almxfl, synalm
+    rng = np.random.default_rng(55)
+    lmaxs = [int(x) for x in rng.choice([255, 511, 1023, 2047, 4095], nreq)]
+    spins = [int(x) for x in rng.choice([0, 2], nreq)]
+    epss = [float(x) for x in rng.choice([1e-5, 1e-7, 1e-10], nreq)]
+    kinds = ["gclm2lenmap" if x else "lenmap2gclm" for x in rng.integers(0, 2, nreq)]
+    defl = {}
+    for L in sorted(set(lmaxs)):
+        ll = np.arange(L + 1, dtype=np.float64)
+        cldd = np.zeros(L + 1)
+        cldd[2:] = ll[2:] ** -2.0
+        cldd *= (2.5 / 60.0 * math.pi / 180.0) ** 2 / (np.sum((2 * ll + 1) * cldd) / (4 * math.pi))
+        with np.errstate(divide="ignore"):
+            fl = np.where(ll > 0, 1.0 / np.sqrt(ll * (ll + 1)), 0.0)
+        defl[L] = torch.as_tensor(almxfl(synalm(cldd, L, rng=rng), fl, L, False), device="cuda")
+    data = []
+    for L, sp, kind in zip(lmaxs, spins, kinds):
+        nc = 1 if sp == 0 else 2
+        if kind == "gclm2lenmap":
+            cl = 1.0 / (np.arange(L + 1) + 1.0) ** 2
+            data.append(torch.as_tensor(np.stack([synalm(cl, L, rng=rng) for _ in range(nc)]), device="cuda"))
+        else:
+            npix = (L + 1) * (2 * L + 2)
+            data.append(torch.as_tensor(rng.standard_normal((nc, npix)), device="cuda"))
+
+    def requests():
+        return [Request(k, L, sp, e, d, defl[L]) for k, L, sp, e, d in zip(kinds, lmaxs, spins, epss, data)]
+
+    offsets = poisson_arrivals(nreq, rate, seed=5)
+    server = LensingServer(hbm_budget_bytes=200 << 30, plans_per_key=1, max_batch=8)
+    cold = serve(server, requests(), offsets)
+    server.finished = []
+    warm = serve(server, requests(), offsets)
+    warm["plan_bytes"] = server.used_bytes
+    server.slots.clear()  # release the served plans before the plain path builds its own
+    del server
+    import gc
+    gc.collect()
+    # plain: one call at a time on resident plans of each configuration
+    plans = {}
+    s = lambda: ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    p = lambda t: ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+    def one(i):
+        L, sp, e, kind = lmaxs[i], spins[i], epss[i], kinds[i]
+        nc = 1 if sp == 0 else 2
+        pl = plans.get((L, sp, e))
+        if pl is None:
+            pl = plans[(L, sp, e)] = LensingPlan(L, sp, e)
+        d = data[i]
+        if kind == "gclm2lenmap":
+            out = torch.empty((2, pl.npix()), dtype=torch.float64, device="cuda")
+            _lib_call("nusht_gclm2lenmap", pl.nuplan._h, pl.ring._h, p(d), p(defl[L]), p(out[0]),
+                      p(out[1]) if sp else None, 1, s())
+        else:
+            res = torch.empty((nc, pl.nalm()), dtype=torch.complex128, device="cuda")
+            _lib_call("nusht_lenmap2gclm", pl.nuplan._h, pl.ring._h, p(d[0]), p(d[1]) if sp else None, p(defl[L]),
+                      p(res), 1, s())
+
+    from cunusht_amd import _lib
+    _lib_call = _lib.call
+    for i in range(nreq):  # build the plans and run each request once
+        one(i)
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for i in range(nreq):
+        ta = time.perf_counter()
+        one(i)
+        torch.cuda.synchronize()
+        lat.append(1e3 * (time.perf_counter() - ta))
+    t_plain = time.perf_counter() - t0
+    mix = {}
+    for L, sp, e, k in zip(lmaxs, spins, epss, kinds):
+        key = f"{L}/{sp}"
+        mix[key] = mix.get(key, 0) + 1
+    return {"metric": f"served lensing requests/s, {nreq} requests of mixed lmax 255..4095, spin 0/2, "
+                      f"eps 1e-5/1e-7/1e-10, forward and adjoint; p50/p99 ms",
+            "unit": "requests/s", "arrival_rate_per_s": rate, "requests": nreq,
+            "served_cold": cold, "served_warm": warm,
+            "plain": {"requests_per_s": nreq / t_plain, "p50_ms": float(np.percentile(lat, 50)),
+                      "p99_ms": float(np.percentile(lat, 99))},
+            "served_warm_over_plain": warm["requests_per_s"] / (nreq / t_plain),
+            "forward": kinds.count("gclm2lenmap"), "adjoint": kinds.count("lenmap2gclm"), "mix_lmax_spin": mix}
+
+
 def geometry_ntheta_ref(lmax: int) -> int:
     from oracle.geometry import reference_cc_sizes
     return reference_cc_sizes(lmax)[0]
@@ -285,6 +386,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=32,
                     help="requests replayed through the serving loop (0 = skip); they share 4 deflections")
     ap.add_argument("--serve-rate", type=float, default=400.0, help="Poisson arrival rate of served requests (1/s)")
+    ap.add_argument("--serve-mix", type=int, default=0,
+                    help="run the mixed serving configuration with this many requests instead of the headline")
     ap.add_argument("--uneven-batch", type=int, default=0,
                     help="instead of the headline: N independent gclm2lenmap jobs of lmax in {511, 1023, 1535, 2047}, "
                          "spin in {0, 2}, eps 1e-7, as one hipGraph-captured batch vs N sequential calls")
@@ -304,6 +407,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     if args.uneven_batch > 0:
         print(json.dumps(uneven_batch(args.uneven_batch, args.steps, args.warmup)))
+        return
+    if args.serve_mix > 0:
+        print(json.dumps(serve_mix(args.serve_mix, args.serve_rate)))
         return
     L, eps, B = args.lmax, args.eps, args.batch
     print(f"[bench] lmax={L} eps={eps:g} batch={B} steps={args.steps} warmup={args.warmup}", file=sys.stderr,
