@@ -504,6 +504,28 @@ def main():
     rate_plain = args.steps / t_plain
     print(f"[bench] batched: {rate_batch:.2f} maps/s", file=sys.stderr, flush=True)
 
+    # the adjoint (lenmap2gclm), one call at a time with its own deflection, like the plain forward path
+    _lib.timing_reset()
+    _lib.timing_enable(True)
+    T.lenmap2gclm(out, dlms[0], None, L, L, spin=2)
+    torch.cuda.synchronize()
+    _lib.timing_reset()
+    evs_a = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        evs_a[i][0].record()
+        T.lenmap2gclm(out, dlms[(i + 1) % nin], None, L, L, spin=2)
+        evs_a[i][1].record()
+    torch.cuda.synchronize()
+    t_adj = time.perf_counter() - t0
+    stages_adj = _lib.timing_read()
+    _lib.timing_enable(False)
+    lat_a = np.array([a.elapsed_time(b) for a, b in evs_a])
+    adjoint = {"value": args.steps / t_adj, "unit": "lenmap2gclm calls/s", "p50_ms": float(np.percentile(lat_a, 50)),
+               "p99_ms": float(np.percentile(lat_a, 99)),
+               "stages_ms_per_call": {k: v[0] / args.steps for k, v in sorted(stages_adj.items())}}
+    print(f"[bench] adjoint: {adjoint['value']:.2f} calls/s", file=sys.stderr, flush=True)
+
     # accuracy of both paths against exact FP64 synthesis at sampled pixels of the deflected grid (the
     # library's point plan: Legendre recursion on the pixels' own colatitudes and a direct sum over m, no nuFFT)
     batched_step(0)  # deflection dlms[0]; map 0 of the batch is gclms[0]
@@ -606,6 +628,7 @@ def main():
         "plain": {"value": rate_plain, "p50_ms": float(np.percentile(lat, 50)),
                   "p99_ms": float(np.percentile(lat, 99))},
         "served": served,
+        "adjoint": adjoint,
         "batched_over_plain": rate_batch / rate_plain,
         "served_over_plain": (served["maps_per_s"] / rate_plain) if served else None,
         "accuracy": accuracy,

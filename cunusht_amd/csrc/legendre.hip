@@ -630,31 +630,73 @@ template <bool SP> struct AdjCfg {
 // Each step exchanges half of the remaining values with lane ^ off and keeps the other half,
 // so after log2(K) steps every lane holds one fully lane-summed element; the remaining steps
 // are plain all-reduce butterflies.  Lane ln ends up with element reduce_index<K>(ln).
+// The exchanges are VALU lane moves, not LDS permutes: lane ^ 32 and lane ^ 16 by the gfx950 permlane
+// swaps (v_permlane32_swap: lanes 32..63 of the first operand with lanes 0..31 of the second;
+// v_permlane16_swap: the odd 16-lane rows of the first with the even rows of the second; with both operands
+// the value itself, the lane's partner value is in one of the two results), lane ^ 8 by a DPP rotation by 8
+// inside each 16-lane row.  The closing all-reduce over the lanes that hold one element (16 for K = 4, 8 for
+// K = 8) adds the row rotation by 8 (16 lanes only), the half-row mirror and the quad swaps 2 and 1.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  int lo = __double2loint(v), hi = __double2hiint(v);
+  lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
+  hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+constexpr int kDppRowRor8 = 0x128, kDppRowHalfMirror = 0x141, kDppQuadXor2 = 0x4E, kDppQuadXor1 = 0xB1;
+
+// the value of lane ^ OFF
+template <int OFF>
+__device__ __forceinline__ double lane_xor(double v) {
+  if constexpr (OFF == 8) {
+    return dpp_mov<kDppRowRor8>(v);
+  } else {
+    static_assert(OFF == 16 || OFF == 32, "lane exchanges at offsets 32, 16 and 8");
+    const unsigned lo = (unsigned)__double2loint(v), hi = (unsigned)__double2hiint(v);
+    const bool first = ((threadIdx.x & 63) & OFF) == 0;
+    if constexpr (OFF == 32) {
+      const auto rl = __builtin_amdgcn_permlane32_swap(lo, lo, false, false);
+      const auto rh = __builtin_amdgcn_permlane32_swap(hi, hi, false, false);
+      return __hiloint2double((int)(first ? rh[1] : rh[0]), (int)(first ? rl[1] : rl[0]));
+    } else {
+      const auto rl = __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+      const auto rh = __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+      return __hiloint2double((int)(first ? rh[1] : rh[0]), (int)(first ? rl[1] : rl[0]));
+    }
+  }
+}
+
 template <int K, int k, int off>
 __device__ __forceinline__ void tr_step(double2 (&v)[K], int ln) {
-  if constexpr (off > 0) {
-    if constexpr (k > 1) {
-      constexpr int half = k / 2;
-      const bool h = (ln & off) != 0;
+  if constexpr (k > 1) {
+    constexpr int half = k / 2;
+    const bool h = (ln & off) != 0;
 #pragma unroll
-      for (int i = 0; i < half; ++i) {
-        double2 keep = h ? v[i + half] : v[i];
-        double2 send = h ? v[i] : v[i + half];
-        double rx = __shfl_xor(send.x, off, 64);
-        double ry = __shfl_xor(send.y, off, 64);
-        v[i] = double2{keep.x + rx, keep.y + ry};
-      }
-      tr_step<K, half, off / 2>(v, ln);
-    } else {
-      v[0].x += __shfl_xor(v[0].x, off, 64);
-      v[0].y += __shfl_xor(v[0].y, off, 64);
-      tr_step<K, 1, off / 2>(v, ln);
+    for (int i = 0; i < half; ++i) {
+      const double2 keep = h ? v[i + half] : v[i];
+      const double2 send = h ? v[i] : v[i + half];
+      v[i] = double2{keep.x + lane_xor<off>(send.x), keep.y + lane_xor<off>(send.y)};
     }
+    tr_step<K, half, off / 2>(v, ln);
+  } else {
+    double x = v[0].x, y = v[0].y;
+    if constexpr (off == 8) {
+      x += dpp_mov<kDppRowRor8>(x);
+      y += dpp_mov<kDppRowRor8>(y);
+    }
+    x += dpp_mov<kDppRowHalfMirror>(x);
+    y += dpp_mov<kDppRowHalfMirror>(y);
+    x += dpp_mov<kDppQuadXor2>(x);
+    y += dpp_mov<kDppQuadXor2>(y);
+    x += dpp_mov<kDppQuadXor1>(x);
+    y += dpp_mov<kDppQuadXor1>(y);
+    v[0] = double2{x, y};
   }
 }
 
 template <int K>
 __device__ __forceinline__ double2 transpose_reduce(double2 (&v)[K]) {
+  static_assert(K == 4 || K == 8, "the lane exchanges cover K = 4 and K = 8");
   tr_step<K, K, 32>(v, (int)(threadIdx.x & 63));
   return v[0];
 }

@@ -72,7 +72,7 @@ class RingSHT:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value:
+        if h is not None and h.value and _lib is not None:  # at interpreter exit the module may be gone
             _lib.load().nusht_ringplan_destroy(h)
             self._h = None
 
@@ -103,7 +103,7 @@ class PointPlan:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value:
+        if h is not None and h.value and _lib is not None:  # at interpreter exit the module may be gone
             _lib.load().nusht_pointplan_destroy(h)
             self._h = None
 
@@ -137,7 +137,7 @@ class NuPlan:
 
     def __del__(self):
         h = getattr(self, "_h", None)
-        if h is not None and h.value:
+        if h is not None and h.value and _lib is not None:  # at interpreter exit the module may be gone
             _lib.load().nusht_plan_destroy(h)
             self._h = None
 
