@@ -363,3 +363,20 @@ def test_adjointness_reference_configuration(spin):
     v1 = almutil.myalmdot(a[0], ad[0], lmax)
     v2 = float(np.real(np.vdot(y, x)))
     np.testing.assert_allclose(v1, v2, rtol=1e-9)
+
+
+@pytest.mark.parametrize("spin", [0, 2])
+def test_lensgclm_identity_at_zero_deflection(spin):
+    """lensgclm = weighted adjoint synthesis of gclm2lenmap (CPU_nufft_transformer.py:948-971): with no
+    deflection it returns the input alm (GL quadrature is exact for band-limited maps)."""
+    L = 16
+    rng = np.random.default_rng(800 + spin)
+    nc = 1 if spin == 0 else 2
+    g = np.atleast_2d(almutil.random_alm(L, spin, nc, rng))
+    T = _transformer(L)
+    zero = np.zeros(almutil.nalm(L), dtype=np.complex128)
+    out = T.lensgclm(g, zero, spin, L, nthreads=2, epsilon=1e-12)
+    assert out.shape == g.shape
+    assert _relmax(out, g) < 1e-10
+    grad = T.lensgclm(g, zero, spin, L, nthreads=2, epsilon=1e-12, out_sht_mode="GRAD_ONLY")
+    assert grad.shape == (1, g.shape[1]) and _relmax(grad[0], g[0]) < 1e-10
