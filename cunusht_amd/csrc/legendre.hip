@@ -790,17 +790,21 @@ __global__ __launch_bounds__(AdjCfg<SP>::MaxThreads) void k_adjoint(AdjArgs a) {
       const int rn = a.slot_north[slot];
       int rs = a.slot_south[slot];
       if (rs == rn) rs = -1;
+      // the south-ring sign (-1)^{l+m} is sgA at even chunk positions and -sgA at odd ones (chunks start at
+      // l0 plus multiples of 4): sgA is folded in here, the alternation into the l-loop's unrolled FMAs
       if (!SP) {
         double2 gn = a.G[(long)m * a.sm + (long)rn * a.sr];
         double2 gs = rs >= 0 ? a.G[(long)m * a.sm + (long)rs * a.sr] : double2{0.0, 0.0};
-        P0[r] = cadd(gn, gs);
-        P1[r] = csub(gn, gs);
+        P0[r] = sgA > 0 ? cadd(gn, gs) : csub(gn, gs);  // taken at even positions
+        P1[r] = sgA > 0 ? csub(gn, gs) : cadd(gn, gs);  // taken at odd positions
       } else {
         const long rowp = (long)(L + m) * a.sm, rowm = (long)(L - m) * a.sm;
         P0[r] = a.G[rowp + (long)rn * a.sr];
         P1[r] = rs >= 0 ? a.G[rowp + (long)rs * a.sr] : double2{0.0, 0.0};
         Q0[r] = cconj(a.G[rowm + (long)rn * a.sr]);
         Q1[r] = rs >= 0 ? cconj(a.G[rowm + (long)rs * a.sr]) : double2{0.0, 0.0};
+        P1[r] = double2{sgA * P1[r].x, sgA * P1[r].y};
+        Q1[r] = double2{sgA * Q1[r].x, sgA * Q1[r].y};
       }
     }
   }
@@ -823,7 +827,6 @@ __global__ __launch_bounds__(AdjCfg<SP>::MaxThreads) void k_adjoint(AdjArgs a) {
     for (int i = 0; i < K; ++i) {
       const int l = lc + i;
       if (l <= L) {
-        const double sg = (i & 1) ? -sgA * (((lc - l0) & 1) ? -1.0 : 1.0) : sgA * (((lc - l0) & 1) ? -1.0 : 1.0);
         const double2 rc = (l < L) ? rec[l - l0] : double2{0.0, 0.0};
 #pragma unroll
         for (int r = 0; r < kAdjR; ++r) {
@@ -831,13 +834,13 @@ __global__ __launch_bounds__(AdjCfg<SP>::MaxThreads) void k_adjoint(AdjArgs a) {
           double wu = mu[r], wv = nv_[r];
           if (slow) { wu = ku[r] == 0.0 ? wu : 0.0; wv = kv[r] == 0.0 ? wv : 0.0; }
           if (!SP) {
-            double2 p = sg > 0 ? P0[r] : P1[r];
-            fma2(X[i], p, wu);
+            fma2(X[i], (i & 1) ? P1[r] : P0[r], wu);
           } else {
+            const double sv = (i & 1) ? -wv : wv, su = (i & 1) ? -wu : wu;  // sign is a compile-time constant
             fma2(X[i], P0[r], wu);
-            fma2(X[i], double2{sg * P1[r].x, sg * P1[r].y}, wv);
+            fma2(X[i], P1[r], sv);
             fma2(Y[i], Q0[r], wv);
-            fma2(Y[i], double2{sg * Q1[r].x, sg * Q1[r].y}, wu);
+            fma2(Y[i], Q1[r], su);
           }
           if (l < L) {
             double t = rc.x * x[r];
