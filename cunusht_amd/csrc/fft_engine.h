@@ -260,8 +260,16 @@ __device__ __forceinline__ void fft_stage(c2_t<T>* buf, int n, int ns, const c2_
 // Forward FFT of buf[0..n) in LDS.  Caller must __syncthreads() after filling buf.
 // GENERIC enables the odd radices 7 and 13 (needed only for externally imposed ring lengths);
 // lengths chosen internally are 2-3-5 smooth and use radices 8, 4, 2, 3, 5.
-template <typename T, int NT, int NMAX, bool GENERIC, bool SI, bool SO>
+// R16 enables radix-16 stages (lengths planned with FftLength::init(n, true)); they cost 64 VGPRs of values per
+// butterfly in FP64, so only kernels with the register room (the 1024-thread DFS passes) turn them on.
+template <typename T, int NT, int NMAX, bool GENERIC, bool SI, bool SO, bool R16 = false>
 __device__ __forceinline__ void fft_stage_r(c2_t<T>* buf, const FftDesc<T>& d, int R, int ns) {
+  if constexpr (R16) {
+    if (R == 16) {
+      fft_stage<T, 16, NT, NMAX, SI, SO>(buf, d.n, ns, d.tw);
+      return;
+    }
+  }
   switch (R) {
     case 2: fft_stage<T, 2, NT, NMAX, SI, SO>(buf, d.n, ns, d.tw); break;
     case 3: fft_stage<T, 3, NT, NMAX, SI, SO>(buf, d.n, ns, d.tw); break;
@@ -279,24 +287,24 @@ __device__ __forceinline__ void fft_stage_r(c2_t<T>* buf, const FftDesc<T>& d, i
 
 // SWZ: lengths that are multiples of 16 keep the stages in between in the bank-spread order (fft_swz);
 // it costs registers, so kernels that are register-bound leave it off.
-template <typename T, int NT, int NMAX, bool GENERIC = false, bool SWZ = false>
+template <typename T, int NT, int NMAX, bool GENERIC = false, bool SWZ = false, bool R16 = false>
 __device__ void fft_lds(c2_t<T>* buf, const FftDesc<T>& d) {
   int ns = 1;
   if constexpr (SWZ) {
     if ((d.n & 15) == 0 && d.nstages > 1) {
       const int last = d.nstages - 1;
-      fft_stage_r<T, NT, NMAX, GENERIC, false, true>(buf, d, d.radix[0], ns);
+      fft_stage_r<T, NT, NMAX, GENERIC, false, true, R16>(buf, d, d.radix[0], ns);
       ns *= d.radix[0];
       for (int s = 1; s < last; ++s) {
-        fft_stage_r<T, NT, NMAX, GENERIC, true, true>(buf, d, d.radix[s], ns);
+        fft_stage_r<T, NT, NMAX, GENERIC, true, true, R16>(buf, d, d.radix[s], ns);
         ns *= d.radix[s];
       }
-      fft_stage_r<T, NT, NMAX, GENERIC, true, false>(buf, d, d.radix[last], ns);
+      fft_stage_r<T, NT, NMAX, GENERIC, true, false, R16>(buf, d, d.radix[last], ns);
       return;
     }
   }
   for (int s = 0; s < d.nstages; ++s) {
-    fft_stage_r<T, NT, NMAX, GENERIC, false, false>(buf, d, d.radix[s], ns);
+    fft_stage_r<T, NT, NMAX, GENERIC, false, false, R16>(buf, d, d.radix[s], ns);
     ns *= d.radix[s];
   }
 }
@@ -306,7 +314,7 @@ class FftLength {
  public:
   FftLength() = default;
   explicit FftLength(int n) { init(n); }
-  void init(int n);
+  void init(int n, bool radix16 = false);  // radix16: plan 16-point stages first (fft_lds<..., R16 = true> only)
   bool valid() const { return n_ > 0; }
   int n() const { return n_; }
   size_t bytes() const { return tw_.bytes(); }
@@ -325,9 +333,9 @@ class FftLength {
   DevBuf tw_;
 };
 
-// Factorization into supported radices (8, 4 first, then 2, 3, 5, 7, 13).
+// Factorization into supported radices (16 when radix16, then 8, 4, then 2, 3, 5, 7, 13).
 // Returns an empty vector if the length has other prime factors.
-std::vector<int> fft_factorize(int n);
+std::vector<int> fft_factorize(int n, bool radix16 = false);
 bool fft_supported(int n);
 bool fft_needs_generic(int n);  // a radix 7, 11 or 13 stage is required
 // Smallest 2-3-5 smooth length >= n (for choosing internal grid sizes).

@@ -5,10 +5,12 @@
 
 namespace nusht {
 
-std::vector<int> fft_factorize(int n) {
+std::vector<int> fft_factorize(int n, bool radix16) {
   std::vector<int> r;
   if (n < 1) return r;
   int m = n;
+  if (radix16)
+    while (m % 16 == 0) { r.push_back(16); m /= 16; }
   const int big[] = {8, 4};
   for (int b : big)
     while (m % b == 0) { r.push_back(b); m /= b; }
@@ -41,8 +43,8 @@ int fft_good_size(int n) {
 }
 
 template <typename T>
-void FftLength<T>::init(int n) {
-  auto rad = fft_factorize(n);
+void FftLength<T>::init(int n, bool radix16) {
+  auto rad = fft_factorize(n, radix16);
   NUSHT_REQUIRE(!rad.empty(), kErrUnsupported, "fft: unsupported length " + std::to_string(n));
   NUSHT_REQUIRE((int)rad.size() <= kFftMaxStages, kErrUnsupported, "fft: too many stages");
   if (rad.size() == 1 && rad[0] == 1) rad.clear();

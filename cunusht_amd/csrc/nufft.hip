@@ -72,7 +72,7 @@ __global__ __launch_bounds__(NT) void k_theta2(ThetaArgs<T> a) {
     buf[r] = T2{(T)v.x, (T)v.y};
   }
   __syncthreads();
-  fft_lds<T, NT, NMAX, false, true>(buf, a.ftd);
+  fft_lds<T, NT, NMAX, false, true, std::is_same<T, float>::value>(buf, a.ftd);
   // deconvolve, pad to nf (conjugated for the inverse transform)
   constexpr int E = (NMAX + NT - 1) / NT;
   T2 v[E];
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(NT) void k_theta2(ThetaArgs<T> a) {
     if (k < a.nf) buf[k] = v[i];
   }
   __syncthreads();
-  fft_lds<T, NT, NMAX, false, true>(buf, a.fnf);
+  fft_lds<T, NT, NMAX, false, true, std::is_same<T, float>::value>(buf, a.fnf);
   // rows of the kept band; spin 0 also fills the conjugate column
   for (int q = threadIdx.x; q < a.nrow; q += NT) {
     int t = a.row_lo + q;
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(NT) void k_phi2(PhiArgs<T> a) {
     buf[k] = v;
   }
   __syncthreads();
-  fft_lds<T, NT, NMAX, false, true>(buf, a.fnf);
+  fft_lds<T, NT, NMAX, false, true, std::is_same<T, float>::value>(buf, a.fnf);
   T2* grow = a.grid + (long)q * a.nf;
   for (int k = threadIdx.x; k < a.nf; k += NT) {
     T2 s = buf[k];
@@ -156,7 +156,7 @@ __global__ __launch_bounds__(NT) void k_phi1(PhiArgs<T> a) {
   const T2* grow = a.grid + (long)q * a.nf;
   for (int k = threadIdx.x; k < a.nf; k += NT) buf[k] = grow[k];
   __syncthreads();
-  fft_lds<T, NT, NMAX, false, true>(buf, a.fnf);
+  fft_lds<T, NT, NMAX, false, true, std::is_same<T, float>::value>(buf, a.fnf);
   T2* trow = a.Tbuf + (long)q * a.nm;
   for (int i = threadIdx.x; i < a.nm; i += NT) {
     int m = i - a.lmax;
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(NT) void k_theta1(ThetaArgs<T> a) {
     buf[t] = a.Tout[(long)q * a.nm + m + a.lmax];
   }
   __syncthreads();
-  fft_lds<T, NT, NMAX, false, true>(buf, a.fnf);
+  fft_lds<T, NT, NMAX, false, true, std::is_same<T, float>::value>(buf, a.fnf);
   constexpr int E = (NMAX + NT - 1) / NT;
   T2 v[E];
   const double fm = a.deconv[am] / a.ntd;
@@ -212,7 +212,7 @@ __global__ __launch_bounds__(NT) void k_theta1(ThetaArgs<T> a) {
     if (r < a.ntd) buf[r] = v[i];
   }
   __syncthreads();
-  fft_lds<T, NT, NMAX, false, true>(buf, a.ftd);
+  fft_lds<T, NT, NMAX, false, true, std::is_same<T, float>::value>(buf, a.ftd);
   // fold back onto the ntheta rings (conjugate back from the inverse trick)
   double2* g = const_cast<double2*>(a.G) + (long)row * a.gsm;
   for (int j = threadIdx.x; j < a.ntheta; j += NT) {
@@ -433,18 +433,20 @@ struct SpreadArgs {
 };
 
 // Exact transpose of k_interp_w: points spread into an LDS copy of the tile (+ halo) with LDS
-// atomics, then the tile is added to the fine grid with global atomics.
+// atomics, then the tile is added to the fine grid with global atomics.  The LDS tile accumulates in
+// FP64 for both grid precisions: ds_add_f64 sustains ~4x the rate of ds_add_f32 on these point
+// patterns (LDS active cycles per instruction 29 vs 176), and the sums lose nothing before the final
+// rounding to the grid's type.
 template <typename T, int W>
 __global__ __launch_bounds__(256) void k_spread_w(SpreadArgs<T> a) {
-  using T2 = c2_t<T>;
   extern __shared__ unsigned char smem_raw[];
-  T* tile = reinterpret_cast<T*>(smem_raw);  // interleaved re, im
+  double* tile = reinterpret_cast<double*>(smem_raw);  // interleaved re, im
   const int tb = blockIdx.x;
   const int s0 = a.tile_start[tb], s1 = a.tile_start[tb + 1];
   if (s0 == s1) return;
   const int tw = a.tc + W, th = a.tr + W;
   const int ncell = tw * th;
-  for (int i = threadIdx.x; i < 2 * ncell; i += blockDim.x) tile[i] = T(0);
+  for (int i = threadIdx.x; i < 2 * ncell; i += blockDim.x) tile[i] = 0.0;
   __syncthreads();
   const int tile_r = tb / a.ntc, tile_c = tb % a.ntc;
   const int r0 = tile_r * a.tr, c0 = tile_c * a.tc;
@@ -470,15 +472,18 @@ __global__ __launch_bounds__(256) void k_spread_w(SpreadArgs<T> a) {
     }
     T kt[W], kp[W];
     ker_weights<T, W>(i0 - ut, j0u - up, a.beta, a.kpoly, a.deg, kt, kp);
-    T* base = tile + 2 * (lr * tw + lc);
+    double kpd[W];
+#pragma unroll
+    for (int j = 0; j < W; ++j) kpd[j] = (double)kp[j];
+    double* base = tile + 2 * (lr * tw + lc);
 #pragma unroll
     for (int i = 0; i < W; ++i) {
-      const T cr = (T)vr * kt[i], ci = (T)vi * kt[i];
-      T* trow = base + 2 * i * tw;
+      const double cr = vr * (double)kt[i], ci = vi * (double)kt[i];
+      double* trow = base + 2 * i * tw;
 #pragma unroll
       for (int j = 0; j < W; ++j) {
-        atomicAdd(&trow[2 * j], cr * kp[j]);
-        atomicAdd(&trow[2 * j + 1], ci * kp[j]);
+        atomicAdd(&trow[2 * j], cr * kpd[j]);
+        atomicAdd(&trow[2 * j + 1], ci * kpd[j]);
       }
     }
   }
@@ -487,7 +492,7 @@ __global__ __launch_bounds__(256) void k_spread_w(SpreadArgs<T> a) {
     const int rr = i / tw, cc = i - rr * tw;
     const int grow = r0 + rr;
     if (grow >= a.nrow) continue;
-    T vr = tile[2 * i], vi = tile[2 * i + 1];
+    const T vr = (T)tile[2 * i], vi = (T)tile[2 * i + 1];
     if (vr == T(0) && vi == T(0)) continue;
     const int gc = (c0 + cc) % a.nf;  // a tile can be wider than a small grid
     T* g = reinterpret_cast<T*>(a.grid + (long)grow * a.nf + gc);
@@ -501,15 +506,14 @@ constexpr int kMaxUnrolledW = 16;
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_spread(SpreadArgs<T> a) {
-  using T2 = c2_t<T>;
   extern __shared__ unsigned char smem_raw[];
-  T* tile = reinterpret_cast<T*>(smem_raw);  // interleaved re, im
+  double* tile = reinterpret_cast<double*>(smem_raw);  // interleaved re, im, FP64 as in k_spread_w
   const int tb = blockIdx.x;
   const int s0 = a.tile_start[tb], s1 = a.tile_start[tb + 1];
   if (s0 == s1) return;
   const int tw = a.tc + a.w, th = a.tr + a.w;
   const int ncell = tw * th;
-  for (int i = threadIdx.x; i < 2 * ncell; i += blockDim.x) tile[i] = T(0);
+  for (int i = threadIdx.x; i < 2 * ncell; i += blockDim.x) tile[i] = 0.0;
   __syncthreads();
   const int tile_r = tb / a.ntc, tile_c = tb % a.ntc;
   const int r0 = tile_r * a.tr, c0 = tile_c * a.tc;
@@ -541,12 +545,12 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs<T> a) {
     for (int j = 0; j < kMaxW; ++j)
       if (j < a.w) kp[j] = WorkT<T>::ker((T)(fp + j) * inv, a.beta);
     for (int i = 0; i < a.w; ++i) {
-      T kt = WorkT<T>::ker((T)(ft + i) * inv, a.beta);
-      T cr = (T)vr * kt, ci = (T)vi * kt;
-      T* trow = tile + 2 * ((lr + i) * tw + lc);
+      const double kt = (double)WorkT<T>::ker((T)(ft + i) * inv, a.beta);
+      const double cr = vr * kt, ci = vi * kt;
+      double* trow = tile + 2 * ((lr + i) * tw + lc);
       for (int j = 0; j < a.w; ++j) {
-        atomicAdd(&trow[2 * j], cr * kp[j]);
-        atomicAdd(&trow[2 * j + 1], ci * kp[j]);
+        atomicAdd(&trow[2 * j], cr * (double)kp[j]);
+        atomicAdd(&trow[2 * j + 1], ci * (double)kp[j]);
       }
     }
   }
@@ -555,7 +559,7 @@ __global__ __launch_bounds__(256) void k_spread(SpreadArgs<T> a) {
     const int rr = i / tw, cc = i - rr * tw;
     const int grow = r0 + rr;
     if (grow >= a.nrow) continue;
-    T vr = tile[2 * i], vi = tile[2 * i + 1];
+    const T vr = (T)tile[2 * i], vi = (T)tile[2 * i + 1];
     if (vr == T(0) && vi == T(0)) continue;
     const int gc = (c0 + cc) % a.nf;  // a tile can be wider than a small grid
     T* g = reinterpret_cast<T*>(a.grid + (long)grow * a.nf + gc);
@@ -852,7 +856,9 @@ NufftPlan::NufftPlan(const NufftParams& p, long npts_max, hipStream_t st) : p_(p
       upload(kpoly_, c);
     }
   }
-  if (p.single) { ftd_f_.init(p.ntd); fnf_f_.init(p.nf); }
+  // FP32 DFS passes take radix-16 stages (one LDS pass fewer on 8192 and 10240); in FP64 the 16 complex values
+  // per thread spill at 1024 threads, so those keep radix 8
+  if (p.single) { ftd_f_.init(p.ntd, true); fnf_f_.init(p.nf, true); }
   else { ftd_d_.init(p.ntd); fnf_d_.init(p.nf); }
   const size_t cb = p.single ? 8 : 16;
   T_.alloc(cb * (size_t)p.nrow * p.nm);
@@ -982,7 +988,7 @@ void NufftPlan::type1_impl(const double* ire, const double* iim, const double* g
     SpreadArgs<T> sa{p.nf, p.w, p.row_lo, p.nrow, p.spin, tile_r_, tile_c_, ntile_c_, (T)p.beta, kpoly_.as<T>(),
                      p.poly_deg, pts_.as<double2>(), idx_.as<int>(), tile_start_.as<int>(), ire, iim, gamma,
                      grid_.as<T2>()};
-    const size_t lds = sizeof(T2) * (size_t)(tile_r_ + p.w) * (tile_c_ + p.w);
+    const size_t lds = sizeof(double2) * (size_t)(tile_r_ + p.w) * (tile_c_ + p.w);  // FP64 accumulation tile
     launch_spread<T, 2>(p.w, dim3(ntile_r_ * ntile_c_), lds, st, sa);
     NUSHT_LAUNCH_CHECK();
   }
