@@ -639,8 +639,8 @@ template <bool SP> struct AdjCfg {
 template <int CTRL>
 __device__ __forceinline__ double dpp_mov(double v) {
   int lo = __double2loint(v), hi = __double2hiint(v);
-  lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, 0xF, 0xF, false);
-  hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, 0xF, 0xF, false);
+  lo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, false);
+  hi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, false);
   return __hiloint2double(hi, lo);
 }
 constexpr int kDppRowRor8 = 0x128, kDppRowHalfMirror = 0x141, kDppQuadXor2 = 0x4E, kDppQuadXor1 = 0xB1;
@@ -666,6 +666,26 @@ __device__ __forceinline__ double lane_xor(double v) {
   }
 }
 
+// a + (a of lane ^ OFF) in the lanes with (lane & OFF) == 0, b + (b of lane ^ OFF) in the others, for OFF = 32
+// or 16: one swap of the register pair per half does it.  permlane32_swap(x, y) leaves x = (own x | y of lane
+// - 32) and y = (x of lane + 32 | own y) over the two halves, permlane16_swap the same over the row pairs, so
+// x + y is the wanted sum in every lane.
+template <int OFF>
+__device__ __forceinline__ double pair_sum(double a, double b) {
+  const unsigned alo = (unsigned)__double2loint(a), ahi = (unsigned)__double2hiint(a);
+  const unsigned blo = (unsigned)__double2loint(b), bhi = (unsigned)__double2hiint(b);
+  static_assert(OFF == 16 || OFF == 32, "pair sums at offsets 32 and 16");
+  if constexpr (OFF == 32) {
+    const auto rl = __builtin_amdgcn_permlane32_swap(alo, blo, false, false);
+    const auto rh = __builtin_amdgcn_permlane32_swap(ahi, bhi, false, false);
+    return __hiloint2double((int)rh[0], (int)rl[0]) + __hiloint2double((int)rh[1], (int)rl[1]);
+  } else {
+    const auto rl = __builtin_amdgcn_permlane16_swap(alo, blo, false, false);
+    const auto rh = __builtin_amdgcn_permlane16_swap(ahi, bhi, false, false);
+    return __hiloint2double((int)rh[0], (int)rl[0]) + __hiloint2double((int)rh[1], (int)rl[1]);
+  }
+}
+
 template <int K, int k, int off>
 __device__ __forceinline__ void tr_step(double2 (&v)[K], int ln) {
   if constexpr (k > 1) {
@@ -673,9 +693,13 @@ __device__ __forceinline__ void tr_step(double2 (&v)[K], int ln) {
     const bool h = (ln & off) != 0;
 #pragma unroll
     for (int i = 0; i < half; ++i) {
-      const double2 keep = h ? v[i + half] : v[i];
-      const double2 send = h ? v[i] : v[i + half];
-      v[i] = double2{keep.x + lane_xor<off>(send.x), keep.y + lane_xor<off>(send.y)};
+      if constexpr (off >= 16) {
+        v[i] = double2{pair_sum<off>(v[i].x, v[i + half].x), pair_sum<off>(v[i].y, v[i + half].y)};
+      } else {
+        const double2 keep = h ? v[i + half] : v[i];
+        const double2 send = h ? v[i] : v[i + half];
+        v[i] = double2{keep.x + lane_xor<off>(send.x), keep.y + lane_xor<off>(send.y)};
+      }
     }
     tr_step<K, half, off / 2>(v, ln);
   } else {
@@ -808,8 +832,22 @@ __global__ __launch_bounds__(AdjCfg<SP>::MaxThreads) void k_adjoint(AdjArgs a) {
       }
     }
   }
-  // group starts are l0 plus multiples of 8, so they fall on chunk boundaries
-  for (int lc = lmin; lc <= L; lc += K) {
+  // A group that never contributes (start past lmax) is emptied, so that the branch-free chunks below may
+  // carry it along; lall is the l from which every other group of the wave has joined.
+  int lall = l0;
+#pragma unroll
+  for (int r = 0; r < kAdjR; ++r) {
+    if (lst[r] > L) {
+      x[r] = mu[r] = pm[r] = ku[r] = nv_[r] = pv[r] = kv[r] = 0.0;
+      P0[r] = P1[r] = Q0[r] = Q1[r] = double2{0.0, 0.0};
+    } else {
+      lall = max(lall, lst[r]);
+    }
+  }
+  // One chunk of K l values.  FAST: every group has joined, no lane is in the scaled range and every l of the
+  // chunk is below lmax, so the chunk runs without branches; otherwise each group, scale and bound is checked.
+  auto chunk = [&](int lc, auto fast_tag) {
+    constexpr bool FAST = decltype(fast_tag)::value;
     double2 X[K], Y[K];
 #pragma unroll
     for (int i = 0; i < K; ++i) X[i] = double2{0.0, 0.0};
@@ -819,20 +857,20 @@ __global__ __launch_bounds__(AdjCfg<SP>::MaxThreads) void k_adjoint(AdjArgs a) {
     bool act[kAdjR];
 #pragma unroll
     for (int r = 0; r < kAdjR; ++r) {
-      act[r] = lc >= lst[r];
-      slow |= act[r] && ((ku[r] < 0.0) || (kv[r] < 0.0));
+      act[r] = FAST || lc >= lst[r];
+      if (!FAST) slow |= act[r] && ((ku[r] < 0.0) || (kv[r] < 0.0));
     }
-    slow = __any(slow);
+    if (!FAST) slow = __any(slow);
 #pragma unroll
     for (int i = 0; i < K; ++i) {
       const int l = lc + i;
-      if (l <= L) {
-        const double2 rc = (l < L) ? rec[l - l0] : double2{0.0, 0.0};
+      if (FAST || l <= L) {
+        const double2 rc = (FAST || l < L) ? rec[l - l0] : double2{0.0, 0.0};
 #pragma unroll
         for (int r = 0; r < kAdjR; ++r) {
           if (!act[r]) continue;
           double wu = mu[r], wv = nv_[r];
-          if (slow) { wu = ku[r] == 0.0 ? wu : 0.0; wv = kv[r] == 0.0 ? wv : 0.0; }
+          if (!FAST && slow) { wu = ku[r] == 0.0 ? wu : 0.0; wv = kv[r] == 0.0 ? wv : 0.0; }
           if (!SP) {
             fma2(X[i], (i & 1) ? P1[r] : P0[r], wu);
           } else {
@@ -842,12 +880,12 @@ __global__ __launch_bounds__(AdjCfg<SP>::MaxThreads) void k_adjoint(AdjArgs a) {
             fma2(Y[i], Q0[r], wv);
             fma2(Y[i], Q1[r], su);
           }
-          if (l < L) {
+          if (FAST || l < L) {
             double t = rc.x * x[r];
             double nu = fma(t + rc.y, mu[r], -pm[r]);
             pm[r] = mu[r]; mu[r] = nu;
             if (SP) { double nn = fma(t - rc.y, nv_[r], -pv[r]); pv[r] = nv_[r]; nv_[r] = nn; }
-            if (slow) {
+            if (!FAST && slow) {
               if (ku[r] < 0.0 && fabs(mu[r]) > kThresh) { mu[r] *= kSmall; pm[r] *= kSmall; ku[r] += 1.0; }
               if (SP && kv[r] < 0.0 && fabs(nv_[r]) > kThresh) { nv_[r] *= kSmall; pv[r] *= kSmall; kv[r] += 1.0; }
             }
@@ -873,7 +911,21 @@ __global__ __launch_bounds__(AdjCfg<SP>::MaxThreads) void k_adjoint(AdjArgs a) {
         }
       }
     }
+  };
+  using T_ = std::true_type;
+  using F_ = std::false_type;
+  // group starts are l0 plus multiples of 8, so they fall on chunk boundaries; a lane leaves the scaled range
+  // for good once its value is unscaled, so the checked phase ends at the first chunk that qualifies
+  int lc = lmin;
+  for (; lc <= L; lc += K) {
+    bool scaled = false;
+#pragma unroll
+    for (int r = 0; r < kAdjR; ++r) scaled |= (ku[r] < 0.0) || (kv[r] < 0.0);
+    if (lc >= lall && lc + K - 1 < L && !__any(scaled)) break;
+    chunk(lc, F_{});
   }
+  for (; lc + K - 1 < L; lc += K) chunk(lc, T_{});
+  for (; lc <= L; lc += K) chunk(lc, F_{});
   }  // pass
   __syncthreads();
 
