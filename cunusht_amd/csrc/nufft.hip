@@ -432,13 +432,39 @@ struct SpreadArgs {
   c2_t<T>* grid;
 };
 
+// Broadcast of lane I of each 16-lane row to the whole row (DPP row_newbcast).
+template <int I>
+__device__ __forceinline__ float row_bcast(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x150 + I, 0xF, 0xF, false));
+}
+template <int I>
+__device__ __forceinline__ double row_bcast(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x150 + I, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x150 + I, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (I < N) {
+    f(std::integral_constant<int, I>{});
+    static_for<I + 1, N>(f);
+  }
+}
+
 // Exact transpose of k_interp_w: points spread into an LDS copy of the tile (+ halo) with LDS
-// atomics, then the tile is added to the fine grid with global atomics.  The LDS tile accumulates in
-// FP64 for both grid precisions: ds_add_f64 sustains ~4x the rate of ds_add_f32 on these point
-// patterns (LDS active cycles per instruction 29 vs 176), and the sums lose nothing before the final
-// rounding to the grid's type.
+// atomics, then the tile is added to the fine grid with global atomics.  Each point is spread by one
+// 16-lane row of a wave, lane j owning footprint column j (W <= 16): the lane evaluates the kernel
+// polynomial of cell j along both axes, takes the row weights of the other lanes by DPP broadcasts and
+// adds W row cells of its column.  The per-point set-up (cell, rotation by gamma, polynomial arguments)
+// runs one point per lane and reaches the spreading rows by lane shuffles.  The W lanes of a point hit W adjacent cells of one tile row per
+// instruction, so the atomics of a point never collide with each other; one lane per point (64 points
+// per instruction, neighbouring points 1.25 cells apart) made most of them collide.  The LDS tile
+// accumulates in FP64 for both grid precisions: ds_add_f64 sustains several times the rate of
+// ds_add_f32 on these patterns, and the sums lose nothing before the final rounding to the grid's type.
 template <typename T, int W>
 __global__ __launch_bounds__(256) void k_spread_w(SpreadArgs<T> a) {
+  static_assert(W <= 16, "one 16-lane row per point");
   extern __shared__ unsigned char smem_raw[];
   double* tile = reinterpret_cast<double*>(smem_raw);  // interleaved re, im
   const int tb = blockIdx.x;
@@ -452,39 +478,67 @@ __global__ __launch_bounds__(256) void k_spread_w(SpreadArgs<T> a) {
   const int r0 = tile_r * a.tr, c0 = tile_c * a.tc;
   const double scale = a.nf / (2.0 * M_PI);
   const double hw = 0.5 * W;
-  for (int p = s0 + threadIdx.x; p < s1; p += blockDim.x) {
-    const double2 tp = a.pts[p];
-    const double ut = tp.x * scale;
-    double up = tp.y * scale;
-    up -= a.nf * floor(up / a.nf);
-    const int i0 = (int)ceil(ut - hw);
-    const int j0u = (int)ceil(up - hw);
-    const int j0 = j0u < 0 ? j0u + a.nf : j0u;
-    const int lr = i0 - a.row_lo - r0, lc = j0 - c0;
-    if (lr < 0 || lr >= a.tr || lc < 0 || lc >= a.tc) continue;  // never for points binned by set_pts
-    const int o = a.idx[p];
-    double vr = a.in_re[o], vi = (a.spin && a.in_im) ? a.in_im[o] : 0.0;
-    if (a.spin && a.gamma) {
-      double sn, cs;
-      sincos(-a.spin * a.gamma[o], &sn, &cs);
-      double xr = vr * cs - vi * sn, xi = vr * sn + vi * cs;
-      vr = xr; vi = xi;
-    }
-    T kt[W], kp[W];
-    ker_weights<T, W>(i0 - ut, j0u - up, a.beta, a.kpoly, a.deg, kt, kp);
-    double kpd[W];
-#pragma unroll
-    for (int j = 0; j < W; ++j) kpd[j] = (double)kp[j];
-    double* base = tile + 2 * (lr * tw + lc);
-#pragma unroll
-    for (int i = 0; i < W; ++i) {
-      const double cr = vr * (double)kt[i], ci = vi * (double)kt[i];
-      double* trow = base + 2 * i * tw;
-#pragma unroll
-      for (int j = 0; j < W; ++j) {
-        atomicAdd(&trow[2 * j], cr * kpd[j]);
-        atomicAdd(&trow[2 * j + 1], ci * kpd[j]);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int row16 = lane >> 4;  // this lane's 16-lane row of the wave
+  const int j = lane & 15;
+  const bool own = j < W;
+  const int jc = own ? j : W - 1;  // lanes past the footprint evaluate the last cell and add nothing
+  // Each wave takes 64 points at a time: lane-per-point set-up (position, rotation), then 16 rounds in
+  // which row r spreads point 4 t + r, its set-up fetched from that lane.
+  for (int pb = s0 + 64 * wave; pb < s1; pb += 64 * (blockDim.x >> 6)) {
+    const int p = pb + lane;
+    int cell = -1;  // lr * tw + lc, or -1: no point
+    double vr = 0.0, vi = 0.0;
+    T ut_ = T(0), up_ = T(0);
+    if (p < s1) {
+      const double2 tp = a.pts[p];
+      const double ut = tp.x * scale;
+      double up = tp.y * scale;
+      up -= a.nf * floor(up / a.nf);
+      const int i0 = (int)ceil(ut - hw);
+      const int j0u = (int)ceil(up - hw);
+      const int j0 = j0u < 0 ? j0u + a.nf : j0u;
+      const int lr = i0 - a.row_lo - r0, lc = j0 - c0;
+      if (lr >= 0 && lr < a.tr && lc >= 0 && lc < a.tc) {  // always, for points binned by set_pts
+        cell = lr * tw + lc;
+        const int o = a.idx[p];
+        vr = a.in_re[o];
+        vi = (a.spin && a.in_im) ? a.in_im[o] : 0.0;
+        if (a.spin && a.gamma) {
+          double sn, cs;
+          sincos(-a.spin * a.gamma[o], &sn, &cs);
+          const double xr = vr * cs - vi * sn, xi = vr * sn + vi * cs;
+          vr = xr; vi = xi;
+        }
+        // polynomial argument u = f + W/2 - 1/2, f = first cell - position, along theta and phi
+        ut_ = (T)((i0 - ut) + 0.5 * W - 0.5);
+        up_ = (T)((j0u - up) + 0.5 * W - 0.5);
       }
+    }
+    for (int t = 0; t < 16; ++t) {
+      const int src = 4 * t + row16;
+      // every lane takes part in the shuffles (a shuffle reads nothing from an inactive lane)
+      const int cl = __shfl(cell, src, 64);
+      const double pr = __shfl(vr, src, 64), pi = __shfl(vi, src, 64);
+      const T ua = __shfl(ut_, src, 64), ub = __shfl(up_, src, 64);
+      if (cl < 0) continue;  // uniform over the 16-lane row
+      const T* c = a.kpoly + (long)a.deg * W + jc;
+      T kt = *c, kp = kt;
+      for (int k = a.deg - 1; k >= 0; --k) {
+        c -= W;
+        kt = fma(kt, ua, *c);
+        kp = fma(kp, ub, *c);
+      }
+      const double cr = pr * (double)kp, ci = pi * (double)kp;
+      double* col = tile + 2 * (cl + jc);
+      static_for<0, W>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        const double kti = (double)row_bcast<i>(kt);
+        if (own) {
+          atomicAdd(&col[2 * i * tw], cr * kti);
+          atomicAdd(&col[2 * i * tw + 1], ci * kti);
+        }
+      });
     }
   }
   __syncthreads();
