@@ -575,3 +575,41 @@ def test_adjointness_reference_configuration(spin):
     v1 = almutil.myalmdot(a[0], ad[0], lmax)
     v2 = float(np.sum(x * y))
     np.testing.assert_allclose(v1, v2, rtol=1e-9)
+
+
+# ------------------------------------------------------------------ every ES kernel width
+# (epsilon, sigma) -> kernel width at lmax 30: FP32 nuFFT for epsilon >= 1e-6 (widths 2..10), FP64 below (7..16
+# through sigma 2, 22 through sigma 1.25 = the runtime-width kernels above 16)
+_WIDTH_CASES = [(0.1, 0.0, 2), (0.03, 0.0, 3), (1e-2, 0.0, 4), (1e-3, 0.0, 5), (3e-4, 0.0, 6), (1e-4, 0.0, 7),
+                (1e-5, 0.0, 8), (3e-6, 0.0, 9), (1e-6, 0.0, 10), (5e-7, 2.0, 7), (1e-7, 2.0, 8), (3e-9, 0.0, 9),
+                (1e-9, 0.0, 10), (1e-10, 0.0, 11), (1e-11, 0.0, 12), (1e-12, 0.0, 13), (1e-13, 0.0, 14),
+                (1e-14, 0.0, 15), (1e-15, 0.0, 16), (1e-14, 1.25, 22)]
+
+
+@pytest.mark.parametrize("eps,sigma,width", _WIDTH_CASES)
+def test_every_kernel_width(eps, sigma, width):
+    """Interpolation and spreading at every unrolled kernel width (and one runtime width), spin 2: nusht2d2 against
+    direct Y_lm summation, nusht2d1 against the direct adjoint and the dot-product test between the two (the
+    spreading is the exact transpose of the interpolation: 1e-9 in FP64, 1e-5 in FP32)."""
+    import cunusht_amd
+    rng = np.random.default_rng(7000 + width)
+    L, spin, npts = 30, 2, 2000
+    theta = np.arccos(rng.uniform(-1, 1, npts))
+    phi = rng.uniform(0, 2 * np.pi, npts)
+    loc = np.stack([theta, phi], 1)
+    a = almutil.random_alm(L, spin, 2, rng)
+    print(f"case eps={eps:g} sigma={sigma} width={width}", flush=True)
+    T = cunusht_amd.get_transformer("GPU")(("gl", {"lmax": L}), nuFFTtype=2, epsilon=eps, spin=spin, sigma=sigma)
+    y = T.nusht2d2(L, L, _t(a), _t(loc), spin=spin).cpu().numpy()
+    widths = {p.info()["kernel_width"] for p in T._plans.values()}
+    assert widths == {width}, widths
+    single = eps >= 1e-6
+    ref = osht.synth_direct(a, L, spin, theta, phi)
+    assert _rel2(y, np.array([ref.real, ref.imag])) < (30 if single else 10) * max(eps, 1e-13)
+    x = rng.uniform(-0.5, 0.5, (2, npts))
+    ad = T.nusht2d1(L, L, _t(x), _t(loc), spin=spin).cpu().numpy().reshape(2, -1)
+    aref = osht.adjoint_direct(x[0] + 1j * x[1], L, spin, theta, phi).reshape(2, -1)
+    assert _rel2(ad, aref) < (30 if single else 10) * max(eps, 1e-12)
+    lhs = float(np.sum(x * y))
+    rhs = sum(almutil.myalmdot(a[c], ad[c], L) for c in range(2))
+    assert abs(lhs - rhs) < (1e-5 if single else 1e-9) * abs(lhs), (lhs, rhs)
