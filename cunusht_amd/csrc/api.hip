@@ -19,7 +19,7 @@
 
 using namespace nusht;
 
-// A second stream per plan for the batched forward lensing operator: the Legendre synthesis needs only the alm
+// A second stream per plan for the forward lensing operator: the Legendre synthesis needs only the alm
 // and the pointing / binning only the deflection, so the first runs on `side` while the second runs on the
 // caller's stream, joined before the interpolation.  Created with the plan, on the plan's device.
 struct ForkJoin {
@@ -382,7 +382,7 @@ void op_gclm2lenmap(nusht_plan_t p, nusht_ringplan_t ptg, const double* gclm, co
   synth_batch(p, 1, gclm, 0, lenmap0, lenmap1, 0, ga, st);
 }
 
-// nmaps lensed maps sharing one deflection: the Legendre synthesis of all maps on the plan's side stream
+// nmaps (>= 1) lensed maps sharing one deflection: the Legendre synthesis of all maps on the plan's side stream
 // overlaps the pointing and binning on `st`; the interpolations follow on `st` once both are done.  Under a
 // graph capture of `st` the steps run in order on `st` instead: a fork from a stream that had itself joined a
 // capture through an event (the batch executor's streams) made hipStreamEndCapture crash, so no fork is
@@ -461,7 +461,7 @@ int nusht_gclm2lenmap(nusht_plan_t p, nusht_ringplan_t ptg, const double* gclm, 
     check_lensing(p, ptg);
     NUSHT_REQUIRE(gclm && dlm_scaled && lenmap0, kErrArg, "null array");
     NUSHT_REQUIRE(p->spin == 0 || lenmap1, kErrArg, "spin > 0 needs two output arrays");
-    op_gclm2lenmap(p, ptg, gclm, dlm_scaled, lenmap0, lenmap1, polrot, S(s));
+    op_gclm2lenmap_n(p, ptg, 1, gclm, 0, dlm_scaled, lenmap0, lenmap1, 0, polrot, S(s));
   });
 }
 

@@ -206,7 +206,9 @@ int nusht_nusht2d1(nusht_plan_t p, const double* in0, const double* in1, const d
 
 /* Lensing.  Replace gclm2lenmap (GPU_nufft_transformer.py:486-578) and lenmap2gclm (:580-673), with the
  * spin-s rotation of the CPU backend (CPU_nufft_transformer.py:356-368, 429-529) when polrot != 0.
- * ptg must be a spin-1 ring plan with the same lmax; its npix must not exceed npts_max. */
+ * ptg must be a spin-1 ring plan with the same lmax; its npix must not exceed npts_max.  gclm2lenmap runs the
+ * Legendre synthesis on a second stream of the plan beside the pointing (joined before the interpolation),
+ * in order on s when s is being captured into a graph. */
 int nusht_gclm2lenmap(nusht_plan_t p, nusht_ringplan_t ptg, const double* gclm, const double* dlm_scaled,
                       double* lenmap0, double* lenmap1, int polrot, nusht_stream_t s);
 int nusht_lenmap2gclm(nusht_plan_t p, nusht_ringplan_t ptg, const double* lenmap0, const double* lenmap1,
