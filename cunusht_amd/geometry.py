@@ -56,14 +56,18 @@ def gauss_legendre_thetas(n: int):
     odd_mid = n % 2 == 1
     if odd_mid:
         th[-1] = 0.5 * np.pi
+    # Newton converges quadratically: once every step is below 1e-14 of its node, one more step leaves the
+    # nodes at rounding level (steps then stall at ulp size instead of reaching zero)
+    last = False
     for _ in range(100):
         p, pm, d, y = _pn_diff(n, th)
         step = p / (n * (d - y * p) / np.sin(th))
         if odd_mid:
             step[-1] = 0.0
         th = th - step
-        if np.max(np.abs(step)) < 1e-17:
+        if last:
             break
+        last = bool(np.max(np.abs(step) / th) < 1e-14)
     p, pm, d, y = _pn_diff(n, th)
     w = 2.0 * np.sin(th) ** 2 / (n * pm) ** 2
     ns = n // 2

@@ -51,7 +51,8 @@ def test_gl_weights_match_oracle():
     g = G.get_geom(("gl", {"lmax": 47}))
     o = ogeom.gl_geometry(47)
     assert np.allclose(g.theta, o.theta, rtol=0, atol=1e-15)
-    assert np.allclose(g.weight, o.weight, rtol=1e-14, atol=0)
+    # nodes agree to the ulp; the weight 2 sin^2(theta) / (n P_{n-1})^2 moves by ~n ulp per ulp of theta
+    assert np.allclose(g.weight, o.weight, rtol=1e-13, atol=0)
 
 
 @pytest.mark.parametrize("spin", [0, 2])
