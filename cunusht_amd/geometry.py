@@ -16,6 +16,7 @@ Methods in MATLAB, clencurt; Waldvogel, BIT 46 (2006) 195).
 """
 from __future__ import annotations
 
+import functools
 import inspect
 
 import numpy as np
@@ -50,6 +51,12 @@ def _pn_diff(n: int, theta: np.ndarray):
 
 def gauss_legendre_thetas(n: int):
     """(theta, weights) of the n-point Gauss-Legendre rule, theta increasing, sum(weights) = 2."""
+    th, w = _gauss_legendre_cached(n)
+    return th.copy(), w.copy()
+
+
+@functools.lru_cache(maxsize=32)
+def _gauss_legendre_cached(n: int):
     nh = (n + 1) // 2
     k = np.arange(1, nh + 1)
     th = np.pi * (k - 0.25) / (n + 0.5)
@@ -71,7 +78,10 @@ def gauss_legendre_thetas(n: int):
     p, pm, d, y = _pn_diff(n, th)
     w = 2.0 * np.sin(th) ** 2 / (n * pm) ** 2
     ns = n // 2
-    return np.concatenate([th, np.pi - th[:ns][::-1]]), np.concatenate([w, w[:ns][::-1]])
+    th, w = np.concatenate([th, np.pi - th[:ns][::-1]]), np.concatenate([w, w[:ns][::-1]])
+    th.setflags(write=False)
+    w.setflags(write=False)
+    return th, w
 
 
 def cc_weights(n: int) -> np.ndarray:
