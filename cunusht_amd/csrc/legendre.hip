@@ -416,49 +416,55 @@ __global__ __launch_bounds__(64 * WPB) void k_synth(SynthArgs a) {
 // Operand layouts of v_mfma_f64_16x16x4_f64 on gfx950 (64 lanes):
 //   A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in register i / 4 of lane j + 16 (i % 4).
 
+// One thread per (block q of 4 l, chain, k, map b): the four columns of map b at l = l0 + 4q + k, one 32-byte
+// store (consecutive threads write consecutive 32 bytes).  Block row y serves m = y and m = lmax - y, whose
+// tables together have about lmax + 2 rows of l, so no block row is idle.
 __global__ void k_prep4(int lmax, int s, int nmaps, int ncomp, const double2* __restrict__ alm, long stride_map,
                         const double* __restrict__ fl, const long* __restrict__ mofs, const long* __restrict__ mblk,
                         const double* __restrict__ cnorm, double* __restrict__ coef) {
-  // one thread per table element (coalesced stores): e = ((q * nc + chain) * 4 + k) * 16 + j
-  const int m = blockIdx.y;
-  const int l0 = l0_of(m, s);
-  if (l0 > lmax) return;
-  const int nl = lmax - l0 + 1;
   const int nc = s ? 2 : 1;
-  const long nel = (long)((nl + 3) >> 2) * nc * 64;
   const long nalm = (long)(lmax + 1) * (lmax + 2) / 2;
-  const long mbase = (long)m * (2 * lmax + 1 - m) / 2;
   const double sg = (s & 1) ? 1.0 : -1.0;  // -(-1)^s
-  double* out = coef + mblk[m] * nc * 64;
-  for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < nel; e += (long)gridDim.x * blockDim.x) {
-    const int j = (int)(e & 15), k = (int)((e >> 4) & 3);
-    const long qc = e >> 6;
-    const int chain = (int)(qc % nc);
-    const long q = qc / nc;
-    const int jj = (int)(4 * q + k);
-    const int l = l0 + jj;
-    const int b = j >> 2, c = j & 3;
-    double v = 0.0;
-    if (l <= lmax && b < nmaps) {
-      double cl = cnorm[mofs[m] + jj];
-      if (fl) cl *= fl[l];
-      const double2* ab = alm + b * stride_map;
-      const double2 ev = ab[mbase + l];
-      const double2 bv = (s != 0 && ncomp > 1) ? ab[nalm + mbase + l] : double2{0.0, 0.0};
-      const double sl = ((l + m) & 1) ? -1.0 : 1.0;
-      double2 z;
-      if (s == 0) {
-        z = double2{cl * ev.x, cl * ev.y};
-      } else {
-        const double2 w{sg * cl * (ev.x - bv.y), sg * cl * (ev.y + bv.x)};
-        const double2 wt{-cl * (ev.x + bv.y), -cl * (-ev.y + bv.x)};
-        // u chain: W, s_l Wt;  v chain: Wt, s_l W
-        z = (chain == 0) == (c < 2) ? w : wt;
+  for (int pass = 0; pass < 2; ++pass) {
+    const int m = pass == 0 ? (int)blockIdx.y : lmax - (int)blockIdx.y;
+    if (pass == 1 && m <= (int)blockIdx.y) break;  // the middle m has no partner
+    const int l0 = l0_of(m, s);
+    if (l0 > lmax) continue;
+    const int nl = lmax - l0 + 1;
+    const long nthr = (long)((nl + 3) >> 2) * nc * 16;
+    const long mbase = (long)m * (2 * lmax + 1 - m) / 2;
+    double* out = coef + mblk[m] * nc * 64;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nthr; t += (long)gridDim.x * blockDim.x) {
+      const int b = (int)(t & 3), k = (int)((t >> 2) & 3);
+      const long qc = t >> 4;
+      const int chain = (int)(qc % nc);
+      const long q = qc / nc;
+      const int jj = (int)(4 * q + k);
+      const int l = l0 + jj;
+      double2 lo{0.0, 0.0}, hi{0.0, 0.0};  // columns c = 0, 1 and c = 2, 3
+      if (l <= lmax && b < nmaps) {
+        double cl = cnorm[mofs[m] + jj];
+        if (fl) cl *= fl[l];
+        const double2* ab = alm + b * stride_map;
+        const double2 ev = ab[mbase + l];
+        const double sl = ((l + m) & 1) ? -1.0 : 1.0;
+        if (s == 0) {
+          lo = double2{cl * ev.x, cl * ev.y};
+          hi = double2{sl * lo.x, sl * lo.y};
+        } else {
+          const double2 bv = ncomp > 1 ? ab[nalm + mbase + l] : double2{0.0, 0.0};
+          const double2 w{sg * cl * (ev.x - bv.y), sg * cl * (ev.y + bv.x)};
+          const double2 wt{-cl * (ev.x + bv.y), -cl * (-ev.y + bv.x)};
+          // u chain: W, s_l Wt;  v chain: Wt, s_l W
+          const double2 z0 = chain == 0 ? w : wt, z1 = chain == 0 ? wt : w;
+          lo = z0;
+          hi = double2{sl * z1.x, sl * z1.y};
+        }
       }
-      const double f = c < 2 ? 1.0 : sl;
-      v = f * ((c & 1) ? z.y : z.x);
+      double2* o = reinterpret_cast<double2*>(out + (((q * nc + chain) * 4 + k) * 16 + b * 4));
+      o[0] = lo;
+      o[1] = hi;
     }
-    out[e] = v;
   }
 }
 
@@ -1124,7 +1130,8 @@ static void legendre_synth_mfma(const LegendreTables& t, int nmaps, int ncomp, c
   {
     StageScope sc(t.n_prep.c_str(), st);
     const int nc = t.spin ? 2 : 1;
-    dim3 grid(ceil_div((L + 4) / 4 * nc * 64, 256), L + 1);
+    // block row y covers m = y and m = L - y; its x extent is sized for m = 0 (the longest table)
+    dim3 grid(ceil_div((L + 4) / 4 * nc * 16, 256), L / 2 + 1);
     hipLaunchKernelGGL(k_prep4, grid, dim3(256), 0, st, L, t.spin, nmaps, ncomp, alm, alm_stride_map, fl,
                        t.mofs.as<long>(), t.mblk.as<long>(), t.cnorm.as<double>(), coef4);
     NUSHT_LAUNCH_CHECK();
