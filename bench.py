@@ -447,8 +447,8 @@ def main():
         plan.pts_key = None
         done = 0
         if not args.split_batch:
-            # one call: pointing and binning on this stream, the Legendre synthesis of the first (up to 8)
-            # maps beside it on the plan's own stream
+            # one call: pointing and binning once, then the matrix-core synthesis and the interpolations of
+            # the first (up to 8) maps
             nb = min(8, B)
             _lib.call("nusht_gclm2lenmap_batch", plan._h, ring._h, nb, P(gb[0]), nalm2, P(d), P(outb[0, 0]),
                       P(outb[0, 1]), 2 * npix, 1, stream())

@@ -19,28 +19,6 @@
 
 using namespace nusht;
 
-// A second stream per plan for the forward lensing operator: the Legendre synthesis needs only the alm
-// and the pointing / binning only the deflection, so the first runs on `side` while the second runs on the
-// caller's stream, joined before the interpolation.  Created with the plan, on the plan's device.
-struct ForkJoin {
-  hipStream_t side = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;
-  void ensure() {
-    if (side) return;
-    NUSHT_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
-    NUSHT_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
-    NUSHT_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
-  }
-  ForkJoin() = default;
-  ForkJoin(const ForkJoin&) = delete;
-  ForkJoin& operator=(const ForkJoin&) = delete;
-  ~ForkJoin() {
-    if (join) (void)hipEventDestroy(join);
-    if (fork) (void)hipEventDestroy(fork);
-    if (side) (void)hipStreamDestroy(side);
-  }
-};
-
 struct nusht_plan_s {
   NufftParams np;
   int lmax = 0, spin = 0, ncomp = 1;
@@ -51,7 +29,6 @@ struct nusht_plan_s {
   int nmrows = 0;
   int max_maps = kMaxMaps;  // maps per batched synthesis the G buffer holds
   static constexpr int kMaxMaps = 8;
-  ForkJoin fj;
   size_t bytes() const { return tab.bytes() + nu->bytes() + coef.bytes() + G.bytes() + ptg.bytes(); }
 };
 
@@ -286,7 +263,6 @@ int nusht_plan_create(const nusht_plan_desc* d, nusht_stream_t s, nusht_plan_t* 
     p->G.alloc(sizeof(double2) * (size_t)p->nmrows * p->np.ntheta * p->max_maps);
     p->ptg.alloc(sizeof(double) * 3 * (size_t)p->npts_max);
     legendre_autotune(p->tab, p->ncomp, p->coef.as<double2>(), p->G.as<double2>(), p->np.ntheta, 1, S(s));
-    p->fj.ensure();  // at creation, so that no stream is created under a caller's graph capture
     NUSHT_HIP(hipStreamSynchronize(S(s)));
     *out = p.release();
   });
@@ -382,34 +358,17 @@ void op_gclm2lenmap(nusht_plan_t p, nusht_ringplan_t ptg, const double* gclm, co
   synth_batch(p, 1, gclm, 0, lenmap0, lenmap1, 0, ga, st);
 }
 
-// nmaps (>= 1) lensed maps sharing one deflection: the Legendre synthesis of all maps on the plan's side stream
-// overlaps the pointing and binning on `st`; the interpolations follow on `st` once both are done.  Under a
-// graph capture of `st` the steps run in order on `st` instead: a fork from a stream that had itself joined a
-// capture through an event (the batch executor's streams) made hipStreamEndCapture crash, so no fork is
-// recorded into a graph.
+// nmaps (>= 1) lensed maps sharing one deflection: the pointing and binning once, then the synthesis of all maps
+// (the matrix-core launch for 3 or more) and their interpolations, in order on `st`.  Running the Legendre
+// synthesis on a second stream beside the pointing gained about 1 % of a batched step (both kernels fill the GPU
+// and neither leaves room for the other on a CU) and made the Legendre kernels' timings include the time they
+// shared the GPU; a fork from a stream that had joined a graph capture through an event also made
+// hipStreamEndCapture crash.
 void op_gclm2lenmap_n(nusht_plan_t p, nusht_ringplan_t ptg, int nmaps, const double* gclm, long alm_stride,
                       const double* dlm_scaled, double* out0, double* out1, long out_stride, int polrot,
                       hipStream_t st) {
-  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
-  NUSHT_HIP(hipStreamIsCapturing(st, &cap));
-  if (cap != hipStreamCaptureStatusNone) {
-    const double* ga = lensing_points(p, ptg, dlm_scaled, polrot, st);
-    synth_batch(p, nmaps, gclm, alm_stride, out0, out1, out_stride, ga, st);
-    return;
-  }
-  p->fj.ensure();
-  hipStream_t ss = p->fj.side;
-  NUSHT_HIP(hipEventRecord(p->fj.fork, st));
-  NUSHT_HIP(hipStreamWaitEvent(ss, p->fj.fork, 0));
-  const long gmap = (long)p->nmrows * p->np.ntheta;
-  legendre_synth(p->tab, nmaps, p->ncomp, reinterpret_cast<const double2*>(gclm), alm_stride / 2, nullptr,
-                 p->coef.as<double2>(), p->G.as<double2>(), p->np.ntheta, 1, gmap, ss);
-  NUSHT_HIP(hipEventRecord(p->fj.join, ss));
   const double* ga = lensing_points(p, ptg, dlm_scaled, polrot, st);
-  NUSHT_HIP(hipStreamWaitEvent(st, p->fj.join, 0));
-  for (int b = 0; b < nmaps; ++b)
-    p->nu->type2(p->G.as<double2>() + b * gmap, p->np.ntheta, ga, out0 + b * out_stride,
-                 out1 ? out1 + b * out_stride : nullptr, st);
+  synth_batch(p, nmaps, gclm, alm_stride, out0, out1, out_stride, ga, st);
 }
 
 void op_lenmap2gclm(nusht_plan_t p, nusht_ringplan_t ptg, const double* lenmap0, const double* lenmap1,
@@ -461,7 +420,7 @@ int nusht_gclm2lenmap(nusht_plan_t p, nusht_ringplan_t ptg, const double* gclm, 
     check_lensing(p, ptg);
     NUSHT_REQUIRE(gclm && dlm_scaled && lenmap0, kErrArg, "null array");
     NUSHT_REQUIRE(p->spin == 0 || lenmap1, kErrArg, "spin > 0 needs two output arrays");
-    op_gclm2lenmap_n(p, ptg, 1, gclm, 0, dlm_scaled, lenmap0, lenmap1, 0, polrot, S(s));
+    op_gclm2lenmap(p, ptg, gclm, dlm_scaled, lenmap0, lenmap1, polrot, S(s));
   });
 }
 

@@ -206,9 +206,7 @@ int nusht_nusht2d1(nusht_plan_t p, const double* in0, const double* in1, const d
 
 /* Lensing.  Replace gclm2lenmap (GPU_nufft_transformer.py:486-578) and lenmap2gclm (:580-673), with the
  * spin-s rotation of the CPU backend (CPU_nufft_transformer.py:356-368, 429-529) when polrot != 0.
- * ptg must be a spin-1 ring plan with the same lmax; its npix must not exceed npts_max.  gclm2lenmap runs the
- * Legendre synthesis on a second stream of the plan beside the pointing (joined before the interpolation),
- * in order on s when s is being captured into a graph. */
+ * ptg must be a spin-1 ring plan with the same lmax; its npix must not exceed npts_max. */
 int nusht_gclm2lenmap(nusht_plan_t p, nusht_ringplan_t ptg, const double* gclm, const double* dlm_scaled,
                       double* lenmap0, double* lenmap1, int polrot, nusht_stream_t s);
 int nusht_lenmap2gclm(nusht_plan_t p, nusht_ringplan_t ptg, const double* lenmap0, const double* lenmap1,
@@ -251,9 +249,8 @@ int nusht_nusht2d2_batch(nusht_plan_t p, int nmaps, const double* alm, int64_t a
 
 /* Batched lensing: nmaps gclm sets (stride alm_stride doubles) lensed by one deflection, i.e. nmaps
  * gclm2lenmap calls sharing dlm_scaled (GPU_nufft_transformer.py:486-578 per map), with the pointing and binning
- * done once.  Outputs at lenmap0 + k * out_stride (and lenmap1).  The Legendre synthesis runs on a stream of
- * the plan's own, forked from and joined back into `s`, beside the pointing on `s`; while `s` is being
- * captured into a graph the steps run in order on `s` instead.  Calls on one plan must not overlap. */
+ * done once and the synthesis of 3 or more maps on the matrix cores.  Outputs at lenmap0 + k * out_stride (and
+ * lenmap1).  Everything runs in order on `s`.  Calls on one plan must not overlap. */
 int nusht_gclm2lenmap_batch(nusht_plan_t p, nusht_ringplan_t ptg, int nmaps, const double* gclm,
                             int64_t alm_stride, const double* dlm_scaled, double* lenmap0, double* lenmap1,
                             int64_t out_stride, int polrot, nusht_stream_t s);

@@ -181,8 +181,8 @@ def test_server_admission_respects_budget():
 
 @pytest.mark.parametrize("spin,eps,nmaps", [(2, 1e-10, 5), (0, 1e-10, 3), (2, 1e-5, 2), (2, 1e-10, 1)])
 def test_gclm2lenmap_batch_equals_single_calls(spin, eps, nmaps):
-    """nusht_gclm2lenmap_batch (pointing once, Legendre synthesis forked beside it) == nmaps gclm2lenmap calls
-    with the same deflection; the call is repeated so that the second fork reuses the plan's side stream."""
+    """nusht_gclm2lenmap_batch (pointing once, one synthesis launch for the maps) == nmaps gclm2lenmap calls
+    with the same deflection; the call is repeated on the same plan."""
     from cunusht_amd import _lib
     from cunusht_amd.batch import LensingPlan
     L = 40
