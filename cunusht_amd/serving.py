@@ -10,7 +10,7 @@ the HBM budget:
   different configurations always do;
 * queued forward requests of one configuration that share a deflection (the same dlm_scaled tensor) are
   admitted together, up to `max_batch`, and run as one pointing + one batched synthesis
-  (nusht_gclm2lenmap_batch: one Legendre recursion for all of their maps, beside the pointing);
+  (nusht_gclm2lenmap_batch: the pointing once, one Legendre recursion for all of their maps);
 * plans stay resident; when a new plan would exceed the budget, idle plans of other configurations are
   released, least recently used first;
 * completion is observed through events (no device-wide synchronization), and every request records its
