@@ -457,10 +457,11 @@ __device__ __forceinline__ void static_for(F&& f) {
 // 16-lane row of a wave, lane j owning footprint column j (W <= 16): the lane evaluates the kernel
 // polynomial of cell j along both axes, takes the row weights of the other lanes by DPP broadcasts and
 // adds W row cells of its column.  The per-point set-up (cell, rotation by gamma, polynomial arguments)
-// runs one point per lane and reaches the spreading rows by lane shuffles.  The W lanes of a point hit W adjacent cells of one tile row per
-// instruction, so the atomics of a point never collide with each other; one lane per point (64 points
-// per instruction, neighbouring points 1.25 cells apart) made most of them collide.  The LDS tile
-// accumulates in FP64 for both grid precisions: ds_add_f64 sustains several times the rate of
+// runs one point per lane and reaches the spreading rows by lane shuffles.  The W lanes of a point hit
+// W adjacent cells of one tile row per instruction, so they never share an address; with one lane per
+// point (64 neighbouring points, 1.25 cells apart, per instruction) bank and address conflicts took
+// about half of the LDS cycles (at lmax 4095, w = 12: 2.4e9 of 4.7e9; 1.3e9 of 3.3e9 now).  The LDS
+// tile accumulates in FP64 for both grid precisions: ds_add_f64 sustains several times the rate of
 // ds_add_f32 on these patterns, and the sums lose nothing before the final rounding to the grid's type.
 template <typename T, int W>
 __global__ __launch_bounds__(256) void k_spread_w(SpreadArgs<T> a) {
