@@ -130,3 +130,23 @@ def test_pbounds():
     assert not pb.contains(np.array([0.31, np.pi])).any()
     assert pb.get_range() == pytest.approx(0.4) and pb == G.pbounds(0.1, 0.4)
     assert G.pbdGeometry(G.get_geom(("healpix", {"nside": 2})), pb).npix() == 48
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 512, 4096])
+def test_gl_rule_is_exact_to_degree_2n_minus_1(n):
+    """The n-point rule integrates every Legendre polynomial of degree < 2n exactly (P_0 to 2, the others to 0);
+    checked at degrees 0, 1, 2, n, 2n - 2 and 2n - 1 by the three-term recurrence at the nodes."""
+    import time
+    t0 = time.perf_counter()
+    th, w = G.gauss_legendre_thetas(n)
+    assert time.perf_counter() - t0 < 5.0  # the Newton loop stops once converged
+    x = np.cos(th)
+    degs = sorted(d for d in {0, 1, 2, n, 2 * n - 2, 2 * n - 1} if 0 <= d < 2 * n)
+    p0, p1 = np.ones_like(x), x.copy()
+    vals = {0: p0, 1: p1}
+    for k in range(1, max(degs)):
+        p0, p1 = p1, ((2 * k + 1) * x * p1 - k * p0) / (k + 1)
+        vals[k + 1] = p1
+    for d in degs:
+        got = float(np.dot(w, vals[d]))
+        assert abs(got - (2.0 if d == 0 else 0.0)) < 1e-13, (n, d, got)
