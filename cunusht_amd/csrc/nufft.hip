@@ -460,7 +460,7 @@ __device__ __forceinline__ void static_for(F&& f) {
 // runs one point per lane and reaches the spreading rows by lane shuffles.  The W lanes of a point hit
 // W adjacent cells of one tile row per instruction, so they never share an address; with one lane per
 // point (64 neighbouring points, 1.25 cells apart, per instruction) bank and address conflicts took
-// about half of the LDS cycles (at lmax 4095, w = 12: 2.4e9 of 4.7e9; 1.3e9 of 3.3e9 now).  The LDS
+// about half of the LDS cycles (at lmax 4095, w = 12: 2.4e9 of 4.7e9; 1.3e9 of 3.6e9 now).  The LDS
 // tile accumulates in FP64 for both grid precisions: ds_add_f64 sustains several times the rate of
 // ds_add_f32 on these patterns, and the sums lose nothing before the final rounding to the grid's type.
 template <typename T, int W>
