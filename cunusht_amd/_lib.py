@@ -69,7 +69,7 @@ SIGNATURES = {
     "nusht_adjoint_doubling_f64": [_vp, _i, _i, _i, _vp, _vp],
     "nusht_adjoint_doubling_f32": [_vp, _i, _i, _i, _vp, _vp],
     "nusht_adjoint_doubling_c128": [_vp, _i, _i, _i, _vp, _vp],
-    "nusht_pointing_f64": [_vp, _vp, _vp, _vp, _i, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
+    "nusht_pointing_f64": [_vp, _vp, _vp, _vp, _i, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
     "nusht_magnification_f64": [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _i, _vp, _vp],
     "nusht_almxfl_c128": [_vp, _i, _i, _vp, _vp, _vp],
     "nusht_ringplan_create": [ctypes.POINTER(RingDesc), _vp, ctypes.POINTER(_vp)],

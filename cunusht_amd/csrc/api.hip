@@ -95,11 +95,12 @@ int nusht_adjoint_doubling_c128(const double* dmap, int ntheta, int nphi, int sp
   });
 }
 int nusht_pointing_f64(const double* theta, const double* phi0, const int64_t* nph, const int64_t* ofs, int nring,
-                       int64_t max_nph, const double* d_theta, const double* d_phi, double* out_theta,
+                       int64_t max_nph, int64_t npix, const double* d_theta, const double* d_phi, double* out_theta,
                        double* out_phi, double* out_gamma, nusht_stream_t s) {
   return guard([&] {
+    NUSHT_REQUIRE(theta && phi0 && nph && ofs && d_theta && d_phi && out_theta && out_phi, kErrArg, "null array");
     pointing(theta, phi0, reinterpret_cast<const long*>(nph), reinterpret_cast<const long*>(ofs), nring, max_nph,
-             d_theta, d_phi, out_theta, out_phi, out_gamma, S(s));
+             npix, d_theta, d_phi, out_theta, out_phi, out_gamma, S(s));
   });
 }
 

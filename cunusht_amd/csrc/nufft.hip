@@ -230,7 +230,7 @@ __global__ __launch_bounds__(NT) void k_theta1(ThetaArgs<T> a) {
 // ------------------------------------------------------------------ interpolation (type 2)
 template <typename T>
 struct InterpArgs {
-  int nf, w, row_lo, nrow, spin, tr, tc, ntc;
+  int nf, w, row_lo, nrow, spin, tr, tc, ntc, ntiles;
   T beta;
   const T* kpoly;  // [(deg + 1)][w] kernel polynomials, one per footprint cell
   int deg;
@@ -269,6 +269,18 @@ __device__ __forceinline__ void ker_weights(double ft, double fp, T beta, const 
   }
 }
 
+// The workgroup after the last tile: points binned past the tiles (footprint outside the formed rows, or
+// not finite; only set_pts_async lets them through) get NaN.
+template <typename T>
+__device__ __forceinline__ void interp_mark_bad(const InterpArgs<T>& a) {
+  const double nan = __builtin_nan("");
+  for (long p = a.tile_start[a.ntiles] + threadIdx.x; p < a.npts; p += blockDim.x) {
+    const int o = a.idx[p];
+    a.out_re[o] = nan;
+    if (a.spin) a.out_im[o] = nan;
+  }
+}
+
 // One workgroup per fine-grid tile: the tile plus its (W-1)-cell halo is staged in LDS once and
 // every point binned to the tile is interpolated from it (fully unrolled W x W footprint).
 template <typename T, int W>
@@ -277,6 +289,10 @@ __global__ __launch_bounds__(256) void k_interp_w(InterpArgs<T> a) {
   extern __shared__ unsigned char smem_raw[];
   T2* tile = reinterpret_cast<T2*>(smem_raw);
   const int tb = blockIdx.x;
+  if (tb == a.ntiles) {
+    interp_mark_bad(a);
+    return;
+  }
   const int s0 = a.tile_start[tb], s1 = a.tile_start[tb + 1];
   if (s0 == s1) return;
   const int tw = a.tc + W, th = a.tr + W;
@@ -347,6 +363,10 @@ __global__ __launch_bounds__(256) void k_interp(InterpArgs<T> a) {
   extern __shared__ unsigned char smem_raw[];
   T2* tile = reinterpret_cast<T2*>(smem_raw);
   const int tb = blockIdx.x;
+  if (tb == a.ntiles) {
+    interp_mark_bad(a);
+    return;
+  }
   const int s0 = a.tile_start[tb], s1 = a.tile_start[tb + 1];
   if (s0 == s1) return;
   const int tw = a.tc + a.w, th = a.tr + a.w;
@@ -660,18 +680,26 @@ __device__ __forceinline__ RunInfo wave_runs(int key, bool valid) {
   return r;
 }
 
+// Tile of a point, or a.ntiles (the bin after the last tile, which no interpolation or spreading tile reads)
+// for a point whose footprint leaves the formed rows or that is not finite.  The range checks come before
+// any float-to-int conversion, which is undefined for NaN and out-of-range values.
 __device__ __forceinline__ int bin_key(const BinArgs& a, double theta, double phi, bool& bad) {
   const double scale = a.nf / (2.0 * M_PI);
   const double hw = 0.5 * a.w;
   const double ut = theta * scale;
   double up = phi * scale;
+  const bool finite = ut > -4.0 * a.nf && ut < 4.0 * a.nf && fabs(phi) < 1e15;  // false for NaN
+  if (!finite) {
+    bad = true;
+    return a.ntiles;
+  }
   up -= a.nf * floor(up / a.nf);
   const int q0 = (int)ceil(ut - hw) - a.row_lo;
   int j0 = (int)ceil(up - hw);
   if (j0 < 0) j0 += a.nf;
   if (j0 >= a.nf) j0 -= a.nf;
-  bad = q0 < 0 || q0 + a.w > a.nrow || !(theta == theta) || !(phi == phi);
-  return bad ? 0 : (q0 / a.tr) * a.ntc + (j0 / a.tc);
+  bad = q0 < 0 || q0 + a.w > a.nrow;
+  return bad ? a.ntiles : (q0 / a.tr) * a.ntc + (j0 / a.tc);
 }
 
 __global__ __launch_bounds__(256) void k_bin_count(BinArgs a, int* bad_count) {
@@ -1015,12 +1043,13 @@ void NufftPlan::type2_impl(const double2* G, long sm, const double* gamma, doubl
     NUSHT_LAUNCH_SIZED(k_phi2, T, p.nf, p.nrow, sizeof(T2) * (size_t)p.nf, pa, st);
   }
   if (npts_ == 0) return;
-  InterpArgs<T> ia{p.nf, p.w, p.row_lo, p.nrow, p.spin, tile_r_, tile_c_, ntile_c_, (T)p.beta, kpoly_.as<T>(),
-                   p.poly_deg, npts_, pts_.as<double2>(), idx_.as<int>(), tile_start_.as<int>(), grid_.as<T2>(),
-                   gamma, ore, oim};
+  const int ntiles = ntile_r_ * ntile_c_;
+  InterpArgs<T> ia{p.nf, p.w, p.row_lo, p.nrow, p.spin, tile_r_, tile_c_, ntile_c_, ntiles, (T)p.beta,
+                   kpoly_.as<T>(), p.poly_deg, npts_, pts_.as<double2>(), idx_.as<int>(), tile_start_.as<int>(),
+                   grid_.as<T2>(), gamma, ore, oim};
   StageScope sc("nufft_interp", st);
   const size_t tlds = sizeof(T2) * (size_t)(tile_r_ + p.w) * (tile_c_ + p.w);
-  launch_interp<T, 2>(p.w, dim3(ntile_r_ * ntile_c_), tlds, st, ia);
+  launch_interp<T, 2>(p.w, dim3(ntiles + 1), tlds, st, ia);  // + 1: the workgroup for out-of-range points
   NUSHT_LAUNCH_CHECK();
 }
 

@@ -48,6 +48,7 @@ void magnification(const double* k, const double* g1, const double* g2, const do
 void almxfl(const double2* alm, int lmax, int ncomp, const double* fl, double2* out, hipStream_t st);
 
 void pointing(const double* theta, const double* phi0, const long* nph, const long* ofs, int nring, long max_nph,
+              long npix,
               const double* dt, const double* dp, double* pt, double* pp, double* gamma, hipStream_t st);
 
 }  // namespace nusht

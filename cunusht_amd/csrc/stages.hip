@@ -73,14 +73,19 @@ __global__ void k_adjoint_doubling(const T* __restrict__ dmap, int ntheta, int n
   map[(long)r * nphi + p] = v;
 }
 
+// Pixels k of a ring with ofs + k outside [0, npix), or k >= max_nph, are not touched: a ring table that
+// does not fit the arrays (wrong integer width, wrong offsets) leaves them unwritten instead of faulting
+// (the reference guards its ring index only, cunusht/c/pointing.cu:202).
 __global__ void k_pointing(const double* __restrict__ theta, const double* __restrict__ phi0,
-                           const long* __restrict__ nph, const long* __restrict__ ofs,
+                           const long* __restrict__ nph, const long* __restrict__ ofs, long max_nph, long npix,
                            const double* __restrict__ dt, const double* __restrict__ dp, double* __restrict__ pt,
                            double* __restrict__ pp, double* __restrict__ gamma) {
   const int ring = blockIdx.y;
-  const long n = nph[ring];
-  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < n; k += (long)gridDim.x * blockDim.x) {
-    const long i = ofs[ring] + k;
+  const long n = nph[ring], o = ofs[ring];
+  if (n <= 0 || o < 0 || o >= npix) return;
+  const long kend = min(min(n, max_nph), npix - o);
+  for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < kend; k += (long)gridDim.x * blockDim.x) {
+    const long i = o + k;
     double th = theta[ring];
     double ph = phi0[ring] + k * (2.0 * M_PI / n);
     deflect_point(th, ph, dt[i], dp[i], pt[i], pp[i], gamma ? &gamma[i] : nullptr);
@@ -176,10 +181,12 @@ void almxfl(const double2* alm, int lmax, int ncomp, const double* fl, double2* 
 }
 
 void pointing(const double* theta, const double* phi0, const long* nph, const long* ofs, int nring, long max_nph,
-              const double* dt, const double* dp, double* pt, double* pp, double* gamma, hipStream_t st) {
-  NUSHT_REQUIRE(nring >= 1, kErrArg, "pointing: nring must be >= 1");
+              long npix, const double* dt, const double* dp, double* pt, double* pp, double* gamma, hipStream_t st) {
+  NUSHT_REQUIRE(nring >= 1 && nring <= 65535, kErrArg, "pointing: nring must be in [1, 65535]");
+  NUSHT_REQUIRE(max_nph >= 1 && npix >= 1, kErrArg, "pointing: max_nph and npix must be >= 1");
   dim3 grid(ceil_div(max_nph, 256), nring);
-  hipLaunchKernelGGL(k_pointing, grid, dim3(256), 0, st, theta, phi0, nph, ofs, dt, dp, pt, pp, gamma);
+  hipLaunchKernelGGL(k_pointing, grid, dim3(256), 0, st, theta, phi0, nph, ofs, max_nph, npix, dt, dp, pt, pp,
+                     gamma);
   NUSHT_LAUNCH_CHECK();
 }
 

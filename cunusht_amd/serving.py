@@ -12,7 +12,10 @@ the HBM budget:
   admitted together, up to `max_batch`, and run as one pointing + one batched synthesis
   (nusht_gclm2lenmap_batch: the pointing once, one Legendre recursion for all of their maps);
 * plans stay resident; when a new plan would exceed the budget, idle plans of other configurations are
-  released, least recently used first;
+  released, least recently used first.  A slot's bytes in the budget are its plans, its batch staging
+  buffer (max_batch gclm, allocated with the slot) and the outputs of one launch (max_batch lensed maps,
+  or one gclm for an adjoint request), which exist while the launch is in flight and then belong to the
+  caller;
 * completion is observed through events (no device-wide synchronization), and every request records its
   arrival, admission and completion times, from which p50 / p99 latency follow.
 """
@@ -68,7 +71,12 @@ class _Slot:
         self.stream = torch.cuda.Stream()  # default priority
         self.busy = False
         self.last_used = time.perf_counter()
-        self.nbytes = self.plan.nbytes()
+        nalm, npix = self.plan.nalm(), self.plan.npix()
+        # batch staging (forward batches of 2 .. max_batch maps gather their gclm here)
+        self.stage = torch.empty((max_batch, ncomp, nalm), dtype=torch.complex128, device="cuda") \
+            if max_batch > 1 else None
+        out_bytes = max(max_batch * 2 * npix * 8, ncomp * nalm * 16)
+        self.nbytes = self.plan.nbytes() + (self.stage.nbytes if self.stage is not None else 0) + out_bytes
 
 
 class LensingServer:
@@ -222,7 +230,7 @@ class LensingServer:
                 r0.result = out[0] if p.spin == 0 else out
             else:
                 nb, nalm, npix = len(reqs), p.nalm(), p.npix()
-                alm = torch.empty((nb, p.ncomp, nalm), dtype=torch.complex128, device="cuda")
+                alm = slot.stage[:nb]  # the slot runs one launch at a time: free again once it is idle
                 for b, r in enumerate(reqs):
                     alm[b].copy_(r.data.reshape(p.ncomp, nalm))
                 out = torch.empty((nb, 2, npix), dtype=torch.float64, device="cuda")

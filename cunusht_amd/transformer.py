@@ -10,14 +10,16 @@ Differences from the reference, all additive:
     CPU backend (cunusht/deflection/CPU_nufft_transformer.py:429-529); spin > 0 maps are (Q, U) rows.
   * one instance serves both directions (the reference plans one nuFFT type per instance).
   * `out` / `pointmap` / `alm` / `lenmap` arguments are written to when given.
-  * the pointing cache is keyed on the tensor's identity and version counter instead of a full
-    equality compare (GPU_nufft_transformer.py:550).
+  * the pointing and point caches compare the caller's array with a device copy of the last one used
+    (the reference's full equality compare, GPU_nufft_transformer.py:417-420,470-473,550-556), so a
+    buffer refilled in place or a recycled temporary is never mistaken for the old points.
 Errors: AssertionError for lmax/mmax/epsilon mismatches (as the reference), ValueError for unknown
 options, nusht NuShtError (RuntimeError) for failures inside the library.
 """
 from __future__ import annotations
 
 import ctypes
+import itertools
 import warnings
 
 import numpy as np
@@ -47,6 +49,33 @@ def _dev(x, dtype, name):
         return x.contiguous()
     warnings.warn(f"{name} is not a device tensor and has been transferred to the device", stacklevel=3)
     return torch.as_tensor(np.asarray(x), device="cuda").to(dtype).contiguous()
+
+
+_GENERATION = itertools.count(1)
+
+
+class _Held:
+    """Device copy of the last array a cache was built from.  `same(t)` is the reference's full equality
+    compare (GPU_nufft_transformer.py:417-420,550-556): one device compare and one stream synchronization,
+    and nothing keyed on addresses, which the caching allocator recycles, or on version counters, which
+    writes through raw pointers do not bump."""
+
+    def __init__(self):
+        self.copy = None
+        self.gen = 0
+
+    def same(self, t: torch.Tensor) -> bool:
+        c = self.copy
+        return c is not None and c.shape == t.shape and c.dtype == t.dtype and bool(torch.equal(c, t))
+
+    def hold(self, t: torch.Tensor) -> int:
+        self.copy = t.detach().clone()
+        self.gen = next(_GENERATION)
+        return self.gen
+
+    def clear(self):
+        self.copy = None
+        self.gen = 0
 
 
 class RingSHT:
@@ -118,7 +147,7 @@ class NuPlan:
         _lib.call("nusht_plan_create", ctypes.byref(d), _stream(), ctypes.byref(h))
         self._h = h
         self.lmax, self.spin, self.ncomp, self.epsilon, self.npts_max = lmax, spin, ncomp, epsilon, npts_max
-        self.pts_key = None
+        self.pts_gen = 0  # generation of the point set last given to set_pts (0: none)
 
     def info(self) -> dict:
         i = _lib.PlanInfo()
@@ -140,6 +169,83 @@ class NuPlan:
         if h is not None and h.value and _lib is not None:  # at interpreter exit the module may be gone
             _lib.load().nusht_plan_destroy(h)
             self._h = None
+
+
+class GPU_deflection:
+    """`deflectionlib` of the GPU transformer (cunusht/deflection/GPU_nufft_transformer.py:42-104, built at
+    :164): the pointing grid's geometry, dlm2pointing, and the SHT on the pointing grid that the reference
+    reaches through `__getattr__` on its SHTns transformer (cunusht/sht/GPU_sht_transformer.py:47-121).
+    Everything runs on the owning transformer's ring plans.  Maps are ring-major in the geometry's order, as
+    every other map of this package (the reference's SHTns grid is theta-contiguous here, a layout its
+    lensed maps do not share)."""
+
+    def __init__(self, owner: "GPU_nusht_transformer"):
+        self._owner = owner
+        self.geominfo = owner.geominfo
+        self.geom = owner.geom
+        self.execmode = "normal"
+
+    def dlm2pointing(self, dlm_scaled, mmax_dlm=None, verbose=0, pointing_theta=None, pointing_phi=None):
+        """GPU_nufft_transformer.py:64-104: deflected (theta, phi), written into pointing_theta / pointing_phi
+        when given; returns the (2, npix) pair."""
+        ptg = self._owner.dlm2pointing(dlm_scaled, mmax_dlm, verbose)
+        for dst, src in ((pointing_theta, ptg[0]), (pointing_phi, ptg[1])):
+            if isinstance(dst, torch.Tensor):
+                dst.copy_(src.reshape(dst.shape))
+        return ptg
+
+    def _lm(self, lmax, mmax):
+        L = self._owner.lmax
+        return L if lmax is None else lmax, L if mmax is None else mmax
+
+    def synthesis(self, gclm, lmax=None, mmax=None, mode=None, nthreads=None):
+        """GPU_sht_transformer.py:47-54: spin-0 map of gclm, flat."""
+        lmax, mmax = self._lm(lmax, mmax)
+        return self._owner.synthesis(gclm, None, lmax, mmax).reshape(-1)
+
+    def synthesis_cupy(self, gclm, out, lmax=None, mmax=None, mode=None, nthreads=None):
+        """GPU_sht_transformer.py:56-62 (the reference ignores `out`; it is written here when it fits)."""
+        lmax, mmax = self._lm(lmax, mmax)
+        return self._owner.synthesis(gclm, out, lmax, mmax)
+
+    def adjoint_synthesis_cupy(self, synthmap, gclm=None, lmax=None, mmax=None, mode=None, nthreads=None, iw=None):
+        """GPU_sht_transformer.py:65-73: SHTns analys (quadrature-weighted adjoint) of synthmap * iw."""
+        lmax, mmax = self._lm(lmax, mmax)
+        if iw is not None:
+            synthmap = _dev(synthmap, torch.float64, "synthmap") * _dev(iw, torch.float64, "iw")
+        return self._owner.adjoint_synthesis(synthmap, lmax, mmax, gclm)
+
+    def synthesis_der1(self, gclm, out=None, nthreads=None):
+        """GPU_sht_transformer.py:76-85: (d/dtheta, d/dphi / sin theta) of sum gclm Y_lm, shape (2, npix)."""
+        return self.synthesis_der1_cupy(gclm)
+
+    def synthesis_der1_cupy(self, gclm, out_theta=None, out_phi=None, nthreads=None):
+        """GPU_sht_transformer.py:88-96 (SHTns synth_grad): the gradient pair, (2, npix); written into
+        out_theta / out_phi when given."""
+        o = self._owner
+        alm = _dev(gclm, torch.complex128, "gclm").reshape(-1)
+        assert alm.numel() == Alm.getsize(o.lmax, o.lmax), "gclm must have the transformer's lmax"
+        res = torch.empty((2, o.npix()), dtype=torch.float64, device="cuda")
+        _lib.call("nusht_sht_synth_grad_gl", o._ring(1)._h, _ptr(alm), _ptr(res[0]), _ptr(res[1]), _stream())
+        for dst, src in ((out_theta, res[0]), (out_phi, res[1])):
+            if isinstance(dst, torch.Tensor):
+                dst.copy_(src.reshape(dst.shape))
+        return res
+
+    def analysis(self, map, lmax=None, mmax=None, nthreads=None, alm=None, mode=None):
+        """GPU_sht_transformer.py:99-105: quadrature analysis of a spin-0 map, shape (1, nalm)."""
+        lmax, mmax = self._lm(lmax, mmax)
+        return self._owner.adjoint_synthesis(map, lmax, mmax, alm).reshape(1, -1)
+
+    def adjoint_synthesis(self, map, **kwargs):
+        """GPU_sht_transformer.py:108-118: analysis of a flat or 2-D spin-0 map."""
+        return self.analysis(map, **kwargs)
+
+    def map2alm(self, m, **kwargs):
+        return self.analysis(m, **kwargs)
+
+    def alm2map(self, gclm, **kwargs):
+        return self.synthesis(gclm, **kwargs)
 
 
 class GPU_nusht_transformer:
@@ -167,8 +273,12 @@ class GPU_nusht_transformer:
         self.timer = diagnostics.Timer()
         self._plans = {}
         self._rings = {}
-        self._ptg_key = None
+        self._src = _Held()   # the ptg or dlm_scaled the current pointing came from
+        self._src_kind = None  # "ptg" or "dlm"
+        self._loc = _Held()   # the last loc of nusht2d2 / nusht2d1
+        self._cur_pts = None  # (theta, phi, generation, checked)
         self._gamma = None
+        self.deflectionlib = GPU_deflection(self)
         if nuFFTtype:
             self.plan(epsilon=epsilon, nuFFTtype=nuFFTtype)
 
@@ -200,19 +310,25 @@ class GPU_nusht_transformer:
         """GPU_nufft_transformer.py:249-251: ptg (2, npts) rows (theta, phi)."""
         ptg = _dev(ptg, torch.float64, "ptg")
         self.ptg = ptg
-        for p in self._plans.values():
-            p.pts_key = None
-        self._points(ptg[0].contiguous(), ptg[1].contiguous(), key=("ptg", ptg.data_ptr(), ptg._version))
-
-    def _points(self, theta, phi, key):
-        self._cur_pts = (theta, phi, key)
+        self._gamma = None
+        gen = self._src.hold(ptg)
+        self._src_kind = "ptg"
+        self._cur_pts = (ptg[0].contiguous(), ptg[1].contiguous(), gen, True)
 
     def _ensure_pts(self, plan: NuPlan):
-        theta, phi, key = self._cur_pts
-        if plan.pts_key != key:
+        theta, phi, gen, checked = self._cur_pts
+        if plan.pts_gen != gen:
             # pointing computed here from dlm_scaled lies in [0, pi] by construction: no host check
-            plan.set_pts(theta, phi, checked=key[0] != "dlm")
-            plan.pts_key = key
+            plan.set_pts(theta, phi, checked=checked)
+            plan.pts_gen = gen
+
+    def _loc_pts(self, plan: NuPlan, loc: torch.Tensor):
+        """Points of nusht2d2 / nusht2d1: set on the plan unless it holds these very values."""
+        if not self._loc.same(loc):
+            self._loc.hold(loc)
+        if plan.pts_gen != self._loc.gen:
+            plan.set_pts(loc[:, 0].contiguous(), loc[:, 1].contiguous())
+            plan.pts_gen = self._loc.gen
 
     # ------------------------------------------------------------------ geometry helpers
     def npix(self) -> int:
@@ -258,10 +374,7 @@ class GPU_nusht_transformer:
         npts = loc.shape[0]
         ncomp = self._ncomp(alm, spin)
         plan = self._plan_for(spin, ncomp, eps, npts)
-        key = ("loc", loc.data_ptr(), loc._version, npts)
-        if plan.pts_key != key:
-            plan.set_pts(loc[:, 0].contiguous(), loc[:, 1].contiguous())
-            plan.pts_key = key
+        self._loc_pts(plan, loc)
         out = self._out(pointmap, spin, npts)
         _lib.call("nusht_nusht2d2", plan._h, _ptr(alm), _ptr(out[0]), _ptr(out[1] if spin else None), None, _stream())
         return out[0] if spin == 0 else out
@@ -276,10 +389,7 @@ class GPU_nusht_transformer:
         vals = self._values(pointmap, spin)
         ncomp = 1 if spin == 0 else (alm.shape[0] if alm is not None and alm.dim() == 2 else 2)
         plan = self._plan_for(spin, ncomp, eps, npts)
-        key = ("loc", loc.data_ptr(), loc._version, npts)
-        if plan.pts_key != key:
-            plan.set_pts(loc[:, 0].contiguous(), loc[:, 1].contiguous())
-            plan.pts_key = key
+        self._loc_pts(plan, loc)
         res = self._alm_out(alm, ncomp)
         _lib.call("nusht_nusht2d1", plan._h, _ptr(vals[0]), _ptr(vals[1]), None, _ptr(res), _stream())
         return res
@@ -344,27 +454,25 @@ class GPU_nusht_transformer:
         return magnification_from_alm(self, dlm, dcl, exact)
 
     def _pointing(self, dlm_scaled, ptg, want_gamma):
+        """Current pointing from `ptg` or `dlm_scaled`; recomputed unless the array equals the one the
+        current pointing came from (GPU_nufft_transformer.py:546-558)."""
         if ptg is not None:
             ptg = _dev(ptg, torch.float64, "ptg")
-            key = ("ptg", ptg.data_ptr(), ptg._version)
-            if self._ptg_key != key:
-                self.ptg = ptg
-                self._gamma = None
-                self._points(ptg[0].contiguous(), ptg[1].contiguous(), key)
-                self._ptg_key = key
+            if self._src_kind != "ptg" or not self._src.same(ptg):
+                self.set_pts(ptg)
             return
         assert dlm_scaled is not None or self.dlm_scaled is not None, "Need to provide dlm_scaled if ptg is None"
         if dlm_scaled is None:
             dlm_scaled = self.dlm_scaled
         dlm = _dev(dlm_scaled, torch.complex128, "dlm_scaled")
-        key = ("dlm", dlm.data_ptr(), dlm._version, want_gamma or self._gamma is not None)
-        if self._ptg_key != key:
+        if self._src_kind != "dlm" or not self._src.same(dlm):
             pg = self.dlm2pointing(dlm, gamma=True)
             self.dlm_scaled = dlm
             self.ptg = pg[:2]
             self._gamma = pg[2].contiguous()
-            self._points(pg[0].contiguous(), pg[1].contiguous(), key)
-            self._ptg_key = key
+            gen = self._src.hold(dlm)
+            self._src_kind = "dlm"
+            self._cur_pts = (pg[0].contiguous(), pg[1].contiguous(), gen, False)
 
     def gclm2lenmap(self, gclm, lmax, mmax, ptg=None, dlm_scaled=None, epsilon=None, lenmap=None, verbose=1,
                     execmode="normal", runid=None, GPUdesc="", spin=None, polrot=True):

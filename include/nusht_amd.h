@@ -61,9 +61,11 @@ int nusht_adjoint_doubling_c128(const double* dmap, int ntheta, int nphi, int sp
  * :510-520) and adds the rotation angle gamma of CPU_nufft_transformer.py:158-160.
  *   theta, phi0 (double[nring]), nph, ofs (int64[nring]): ring table; d_theta, d_phi (double[npix]):
  *   spin-1 deflection; out_theta, out_phi (double[npix]); out_gamma may be NULL.
- *   max_nph: host value, the largest entry of nph. */
+ *   max_nph: host value, the largest entry of nph; npix: host value, the length of the pixel arrays.
+ *   Pixels ofs[r] + k outside [0, npix) (or k >= max_nph) are skipped, so a ring table that does not
+ *   match the arrays leaves pixels unwritten rather than writing out of bounds. */
 int nusht_pointing_f64(const double* theta, const double* phi0, const int64_t* nph, const int64_t* ofs,
-                       int nring, int64_t max_nph, const double* d_theta, const double* d_phi,
+                       int nring, int64_t max_nph, int64_t npix, const double* d_theta, const double* d_phi,
                        double* out_theta, double* out_phi, double* out_gamma, nusht_stream_t s);
 
 /* Determinant of the lensing magnification matrix per pixel.  Replaces the pointwise part of

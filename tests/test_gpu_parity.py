@@ -10,7 +10,7 @@ import numpy as np
 import pytest
 import torch
 
-from oracle import almutil, clib, geometry as ogeom, pipeline, pointing as opointing, sht as osht
+from oracle import almutil, clib, geometry as ogeom, pipeline, pointing as opointing, sht as osht, transport
 
 pytestmark = pytest.mark.gpu
 
@@ -124,7 +124,7 @@ def test_pointing_kernel():
     dt[::7] *= 400  # some large deflections take the exact sin(a)/a branch
     o = [torch.empty(th.size, dtype=torch.float64, device="cuda") for _ in range(3)]
     _lib().call("nusht_pointing_f64", _p(_t(geom.theta)), _p(_t(geom.phi0)), _p(_t(geom.nph)), _p(_t(geom.ofs)),
-                geom.theta.size, int(geom.nph.max()), _p(_t(dt)), _p(_t(dp)), _p(o[0]), _p(o[1]), _p(o[2]),
+                geom.theta.size, int(geom.nph.max()), th.size, _p(_t(dt)), _p(_t(dp)), _p(o[0]), _p(o[1]), _p(o[2]),
                 _stream())
     _sync()
     tp, pp = opointing.deflected_angles(th, ph, dt, dp)
@@ -357,6 +357,34 @@ def test_gclm2lenmap_vs_oracle(spin):
     ad = T.lenmap2gclm(_t(x), _t(dlm), None, L, L, spin=spin).cpu().numpy().reshape(nc, -1)
     refad = pipeline.lenmap2gclm(x, dlm, L, spin, nc, rings, 1e-13)
     assert _relmax(ad, refad) < 1e-9
+
+
+@pytest.mark.parametrize("spin", [1, 2])
+@pytest.mark.parametrize("amp", [1e-3, 3e-2])
+def test_gclm2lenmap_rotation_by_parallel_transport(spin, amp):
+    """The library's spin-s lensed map of a constant-ambient-tensor field (band-limited at l = s) equals its
+    exact lensing by parallel transport along each pixel's deflection geodesic (oracle/transport.py): this
+    pins the pointing, the angle gamma and the direction of the e^{i s gamma} rotation
+    (CPU_nufft_transformer.py:158-160,356-368) without the oracle's formulas."""
+    import cunusht_amd
+    rng = np.random.default_rng(170 + 10 * spin + int(amp * 1e4))
+    L = 24
+    rings = ogeom.gl_geometry(L)
+    Tn = transport.random_tensor(spin, rng)
+    th, ph = rings.pixel_angles()
+    w = np.repeat(rings.weight, rings.nph)
+    EB = osht.adjoint_direct(transport.ambient_field(spin, Tn, th, ph) * w, L, spin, th, ph)
+    ll = np.arange(L + 1)
+    dlm = almutil.synalm(np.where(ll >= 2, amp / (ll + 1.0) ** 2, 0.0), L, rng)
+    _, _, _, (dt, dp) = pipeline.dlm2pointing(dlm, L, rings)
+    exact = transport.lensed_ambient_field(spin, Tn, th, ph, dt, dp)
+    T = cunusht_amd.get_transformer("GPU")(("gl", {"lmax": L}), nuFFTtype=2, epsilon=1e-12, spin=spin)
+    print(f"case rotation by transport spin={spin} amp={amp}", flush=True)
+    got = T.gclm2lenmap(_t(EB), L, L, dlm_scaled=_t(dlm), spin=spin).cpu().numpy()
+    _sync()
+    err = np.max(np.abs(got[0] + 1j * got[1] - exact)) / np.max(np.abs(exact))
+    print(f"  max rel err vs parallel transport {err:.2e}")
+    assert err < 1e-9
 
 
 # ------------------------------------------------------------------ batched synthesis

@@ -165,6 +165,30 @@ def test_server_results_same_alone_and_in_mix():
         assert _relmax(got.reshape(ref.shape), ref) < tol, (r.kind, r.lmax, r.spin)
 
 
+def test_server_budget_counts_staging_and_outputs():
+    """A slot's bytes include its batch staging and one launch's outputs: with a budget that holds the plans
+    of two slots but not their staging, a configuration gets one slot, and every request still completes."""
+    import cunusht_amd
+    from cunusht_amd.batch import LensingPlan
+    from cunusht_amd.serving import LensingServer, Request, serve
+    from cunusht_amd.utils import Alm
+    L, spin, eps = 40, 2, 1e-10
+    plan_bytes = LensingPlan(L, spin, eps, max_maps=4).nbytes()
+    nalm, npix = Alm.getsize(L, L), cunusht_amd.get_geom(("gl", {"lmax": L})).npix()
+    slot_bytes = plan_bytes + 4 * 2 * nalm * 16 + max(4 * 2 * npix * 8, 2 * nalm * 16)
+    rng = np.random.default_rng(61)
+    reqs = [Request("gclm2lenmap", L, spin, eps, _t(almutil.random_alm(L, spin, 2, rng)), _t(_deflection(L, rng)))
+            for _ in range(5)]
+    server = LensingServer(hbm_budget_bytes=2 * plan_bytes + 1024, plans_per_key=2, max_batch=4)
+    stats = serve(server, reqs, np.zeros(len(reqs)))
+    assert 2 * plan_bytes + 1024 < 2 * slot_bytes
+    assert stats["requests"] == 5 and stats["plans"] == 1
+    assert server.used_bytes == slot_bytes
+    for r in reqs:
+        ref = _alone(server.slots[r.key][0].plan, "gclm2lenmap", r.data, r.dlm_scaled)
+        assert _relmax(r.result.cpu().numpy(), ref) < 1e-12
+
+
 def test_server_admission_respects_budget():
     """With a budget that holds one plan, two configurations alternate: idle plans are released."""
     from cunusht_amd.serving import LensingServer, Request, serve

@@ -9,7 +9,7 @@ import mpmath as mp
 import numpy as np
 import pytest
 
-from oracle import almutil, clib, geometry, nufft, pipeline, pointing, sht
+from oracle import almutil, clib, geometry, nufft, pipeline, pointing, sht, transport
 
 
 def _wigner_mp(j, m1, m2, b):
@@ -226,6 +226,63 @@ def test_pipeline_vs_direct_and_adjoint(spin):
     lhs = np.sum(x * out)
     rhs = sum(almutil.myalmdot(g[c], ad[c], L) for c in range(nc))
     assert abs(lhs - rhs) < 1e-9 * abs(lhs)  # test/test_adjointOperator.py:268 rtol
+
+
+def _ambient_alm(spin, Tn, L, rings):
+    """E/B of a constant-ambient-tensor field by exact quadrature analysis on the GL rings."""
+    th, ph = rings.pixel_angles()
+    w = np.repeat(rings.weight, rings.nph)
+    return sht.adjoint_direct(transport.ambient_field(spin, Tn, th, ph) * w, L, spin, th, ph)
+
+
+@pytest.mark.parametrize("spin", [1, 2])
+def test_ambient_tensor_field_is_spin_s_at_l_s(spin):
+    """v . m and m^T T m / 2 with m = e_theta + i e_phi are spin +s fields with power at l = s only (the
+    conjugate choice is spin -s and is not reproduced): fixes which field the transport check lenses."""
+    rng = np.random.default_rng(40 + spin)
+    L = 8
+    rings = geometry.gl_geometry(L)
+    Tn = transport.random_tensor(spin, rng)
+    EB = _ambient_alm(spin, Tn, L, rings)
+    ll = np.concatenate([np.arange(m, L + 1) for m in range(L + 1)])
+    assert np.max(np.abs(EB[:, ll != spin])) < 1e-14 * np.max(np.abs(EB))
+    t = np.arccos(rng.uniform(-1, 1, 40))
+    p = rng.uniform(0, 2 * np.pi, 40)
+    f = transport.ambient_field(spin, Tn, t, p)
+    assert np.max(np.abs(sht.synth_direct(EB, L, spin, t, p) - f)) < 1e-13 * np.max(np.abs(f))
+    # the conjugate projection is not a spin +s field
+    th, ph = rings.pixel_angles()
+    w = np.repeat(rings.weight, rings.nph)
+    fbar = np.conj(transport.ambient_field(spin, Tn, th, ph))
+    EBbar = sht.adjoint_direct(fbar * w, L, spin, th, ph)
+    assert np.max(np.abs(sht.synth_direct(EBbar, L, spin, t, p) - np.conj(f))) > 0.1 * np.max(np.abs(f))
+
+
+@pytest.mark.parametrize("spin", [1, 2])
+@pytest.mark.parametrize("amp", [1e-3, 3e-2])
+def test_rotation_direction_by_parallel_transport(spin, amp):
+    """pipeline.gclm2lenmap (pointing, gamma and e^{+i s gamma}, CPU_nufft_transformer.py:158-160,356-368)
+    equals the exact lensing of a constant-ambient-tensor field by parallel transport along each pixel's
+    deflection geodesic; the opposite rotation (e^{-i s gamma}) and no rotation both fail."""
+    rng = np.random.default_rng(50 + 10 * spin + int(amp * 1e4))
+    L = 12
+    rings = geometry.gl_geometry(L)
+    Tn = transport.random_tensor(spin, rng)
+    EB = _ambient_alm(spin, Tn, L, rings)
+    ll = np.arange(L + 1)
+    dlm = almutil.synalm(np.where(ll >= 2, amp / (ll + 1.0) ** 2, 0.0), L, rng)
+    th, ph = rings.pixel_angles()
+    _, _, gam, (dt, dp) = pipeline.dlm2pointing(dlm, L, rings)
+    exact = transport.lensed_ambient_field(spin, Tn, th, ph, dt, dp)
+    scale = np.max(np.abs(exact))
+    QU = pipeline.gclm2lenmap(EB, dlm, L, spin, rings, 1e-12)
+    assert np.max(np.abs(QU[0] + 1j * QU[1] - exact)) < 1e-11 * scale
+    unrot = pipeline.gclm2lenmap(EB, dlm, L, spin, rings, 1e-12, polrot=False)
+    unrot = unrot[0] + 1j * unrot[1]
+    flipped = unrot * np.exp(-1j * spin * gam)  # a planted sign mistake in the rotation
+    tiny = 1e-3 * np.max(np.abs(gam)) * scale
+    assert np.max(np.abs(flipped - exact)) > tiny
+    assert np.max(np.abs(unrot - exact)) > tiny
 
 
 # ------------------------------------------------------------------ magnification (dlm2A)
