@@ -41,9 +41,11 @@ def test_timing_mode_json(tmp_path, monkeypatch):
     g = almutil.random_alm(L, 2, 2, rng)
     dlm_np = _deflection(L, rng)
     dlm = _t(dlm_np)
+    T0 = cunusht_amd.get_transformer("GPU")(("gl", {"lmax": L}), nuFFTtype=2, epsilon=1e-10, spin=2)
+    plain = T0.gclm2lenmap(_t(g), L, L, dlm_scaled=dlm).cpu().numpy()
+    # a fresh transformer: the pointing is computed inside the timed call (a transformer that has seen
+    # these deflection values already would reuse its pointing, as the reference's equality compare does)
     T = cunusht_amd.get_transformer("GPU")(("gl", {"lmax": L}), nuFFTtype=2, epsilon=1e-10, spin=2)
-    plain = T.gclm2lenmap(_t(g), L, L, dlm_scaled=dlm).cpu().numpy()
-    # a new tensor with the same deflection: the pointing is recomputed inside the timed call
     timed = T.gclm2lenmap(_t(g), L, L, dlm_scaled=_t(dlm_np), execmode="timing", runid=3,
                           GPUdesc="mi355x").cpu().numpy()
     assert np.array_equal(plain, timed)
