@@ -106,73 +106,49 @@ def profiled_traffic(stage: str, maps_per_launch: int):
     return None
 
 
-def cpu_baseline(lmax_sample: int, lmax: int, eps: float):
-    """Oracle (C/OpenMP FP64 restatement of the reference CPU algorithm) on a bounded sample:
-    one spin-2 gclm2lenmap at lmax_sample; Legendre stages scaled to `lmax` by N_lm * nrings,
-    the other stages by the pixel count."""
-    from oracle import clib, geometry as ogeom, pipeline
-    from oracle.almutil import almxfl
+def cpu_baseline(lmax: int, eps: float):
+    """Oracle (C/OpenMP FP64 restatement of the reference CPU algorithm, oracle/pipeline.py) timed on the
+    host's cores: one spin-2 gclm2lenmap of the benchmark workload itself (lmax, eps, GL (lmax+1) x
+    (2 lmax + 1), the same input spectra), spin-1 pointing included.  No scaling: the value is 1 / that time."""
+    from oracle import geometry as ogeom, pipeline
     rng = np.random.default_rng(2024)
-    gclm, dlm_scaled = synth_inputs(lmax_sample, rng)
-    rings = ogeom.gl_geometry(lmax_sample, nphi=2 * lmax_sample + 1)
-    L = lmax_sample
+    gclm, dlm_scaled = synth_inputs(lmax, rng)
+    rings = ogeom.gl_geometry(lmax, nphi=2 * lmax + 1)
     t0 = time.perf_counter()
-    ll = np.arange(L + 1, dtype=np.float64)
-    dlm = almxfl(dlm_scaled, np.sqrt(ll * (ll + 1)), L)
-    ta = time.perf_counter()
-    clib.sht_synth_rings(dlm[None, :], L, 1, rings.theta)
-    t_leg1 = time.perf_counter() - ta
-    op = pipeline.NuSHT(L)
-    ta = time.perf_counter()
-    clib.sht_synth_rings(gclm, L, 2, op.thetas)
-    t_leg2 = time.perf_counter() - ta
-    pipeline.gclm2lenmap(gclm, dlm_scaled, L, 2, rings, eps)
-    t_total = time.perf_counter() - t0 - t_leg1 - t_leg2  # full chain (includes one pass of each Legendre)
-    t_leg = t_leg1 + t_leg2
-    t_rest = max(t_total - t_leg, 0.0)
-    nlm_s, nlm = (L + 1) * (L + 2) // 2, (lmax + 1) * (lmax + 2) // 2
-    nt_s = geometry_ntheta_ref(L)
-    nt = geometry_ntheta_ref(lmax)
-    f_leg = (nlm * (lmax + 1 + nt)) / (nlm_s * (L + 1 + nt_s))
-    f_rest = ((lmax + 1) * (2 * lmax + 1)) / ((L + 1) * (2 * L + 1))
-    t_scaled = t_leg * f_leg + t_rest * f_rest
-    return {"value": 1.0 / t_scaled, "unit": "lensed maps/s", "cores": len(os.sched_getaffinity(0)),
-            "kind": "port",
-            "sample": f"oracle gclm2lenmap spin-2 lmax={L} eps={eps:g}, 1 map on the GL {L + 1}x{2 * L + 1} grid "
-                      f"({time.perf_counter() - t0:.1f} s of wall time in all; per map: Legendre {t_leg:.1f} s "
-                      f"scaled x{f_leg:.1f} by N_lm * rings, rest {t_rest:.1f} s scaled x{f_rest:.1f} by pixels)"}
+    pipeline.gclm2lenmap(gclm, dlm_scaled, lmax, 2, rings, eps)
+    t = time.perf_counter() - t0
+    return {"value": 1.0 / t, "unit": "lensed maps/s", "cores": _host_threads(), "kind": "port",
+            "sample": f"oracle gclm2lenmap spin-2 lmax={lmax} eps={eps:g}, one map on the GL {lmax + 1}x{2 * lmax + 1} "
+                      f"grid (the benchmark workload, pointing included), {t:.1f} s"}
 
 
-def cpu_backend_rate(lmax_sample: int, lmax: int, eps: float):
+def cpu_backend_rate(lmax: int, eps: float):
     """This package's CPU backend (get_transformer('CPU', solver='duccnufft'), libnusht_cpu.so) on the same
-    bounded sample as the oracle baseline, scaled the same way: Legendre stages by N_lm * rings, the rest by
-    pixels.  Informational: the product's host path beside the GPU, not the baseline."""
+    workload as the oracle baseline: one spin-2 gclm2lenmap at the benchmark's lmax and eps, pointing
+    included, unscaled.  Informational: the product's host path beside the GPU, not the baseline."""
     import cunusht_amd
     from cunusht_amd.utils import almxfl
     rng = np.random.default_rng(2024)
-    gclm, dlm_scaled = synth_inputs(lmax_sample, rng)
-    L = lmax_sample
-    ll = np.arange(L + 1, dtype=np.float64)
+    gclm, dlm_scaled = synth_inputs(lmax, rng)
+    ll = np.arange(lmax + 1, dtype=np.float64)
     dlm = almxfl(dlm_scaled, np.sqrt(ll * (ll + 1)))
-    T = cunusht_amd.get_transformer("CPU", solver="duccnufft")(("gl", {"lmax": L, "nphi": 2 * L + 1}), epsilon=eps)
-    from cunusht_amd.cpu_transformer import synth_rings
-    nth = len(os.sched_getaffinity(0))
+    T = cunusht_amd.get_transformer("CPU", solver="duccnufft")(("gl", {"lmax": lmax, "nphi": 2 * lmax + 1}),
+                                                                epsilon=eps)
+    nth = _host_threads()
     t0 = time.perf_counter()
-    T.gclm2lenmap(gclm, dlm, L, L, 2, eps, nth, verbose=0)  # pointing included, as in the oracle sample
-    t_total = time.perf_counter() - t0
-    ta = time.perf_counter()  # the spin-1 Legendre stage of the pointing, timed on its own
-    synth_rings(dlm[None, :], L, 1, T.geom.theta, nth)
-    t_leg1 = time.perf_counter() - ta
-    t_leg = T.timer.keys.get("synthesis", 0.0) + t_leg1
-    t_rest = max(t_total - t_leg, 0.0)
-    nlm_s, nlm = (L + 1) * (L + 2) // 2, (lmax + 1) * (lmax + 2) // 2
-    nt_s, nt = geometry_ntheta_ref(L), geometry_ntheta_ref(lmax)
-    f_leg = (nlm * (lmax + 1 + nt)) / (nlm_s * (L + 1 + nt_s))
-    f_rest = ((lmax + 1) * (2 * lmax + 1)) / ((L + 1) * (2 * L + 1))
-    return {"value": 1.0 / (t_leg * f_leg + t_rest * f_rest), "unit": "lensed maps/s", "cores": nth,
-            "sample": f"CPU backend gclm2lenmap spin-2 lmax={L} eps={eps:g} on the GL {L + 1}x{2 * L + 1} grid, "
-                      f"{t_total:.1f} s (Legendre, spin-1 pointing + spin-2 CC, {t_leg:.1f} s scaled x{f_leg:.1f}, "
-                      f"rest {t_rest:.1f} s scaled x{f_rest:.1f})"}
+    T.gclm2lenmap(gclm, dlm, lmax, lmax, 2, eps, nth, verbose=0)
+    t = time.perf_counter() - t0
+    return {"value": 1.0 / t, "unit": "lensed maps/s", "cores": nth,
+            "sample": f"CPU backend gclm2lenmap spin-2 lmax={lmax} eps={eps:g}, one map on the GL "
+                      f"{lmax + 1}x{2 * lmax + 1} grid, pointing included, {t:.1f} s"}
+
+
+def _host_threads() -> int:
+    """Host threads the CPU legs use: the OpenMP thread count when set (16 per GPU on the GPU hosts),
+    else the cores this process may run on."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    return min(n, int(env)) if env and env.isdigit() and int(env) > 0 else n
 
 
 def serve_headline(args, L, eps, gclms, dlms, rng):
@@ -366,11 +342,6 @@ def serve_mix(nreq: int, rate: float):
             "forward": kinds.count("gclm2lenmap"), "adjoint": kinds.count("lenmap2gclm"), "mix_lmax_spin": mix}
 
 
-def geometry_ntheta_ref(lmax: int) -> int:
-    from oracle.geometry import reference_cc_sizes
-    return reference_cc_sizes(lmax)[0]
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -382,7 +353,6 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--split-batch", action="store_true",
                     help="batched step as pointing, then nusht2d2_batch (no overlap of pointing and Legendre)")
-    ap.add_argument("--cpu-lmax", type=int, default=2559)
     ap.add_argument("--serve-requests", type=int, default=32,
                     help="requests replayed through the serving loop (0 = skip); they share 4 deflections")
     ap.add_argument("--serve-rate", type=float, default=400.0, help="Poisson arrival rate of served requests (1/s)")
@@ -444,7 +414,7 @@ def main():
 
     def batched_step(i):
         d = dlms[i % nin]
-        plan.pts_key = None
+        plan.pts_gen = 0  # the batch entry sets the plan's points itself
         done = 0
         if not args.split_batch:
             # one call: pointing and binning once, then the matrix-core synthesis and the interpolations of
@@ -597,9 +567,10 @@ def main():
     cpu = cpu_be = None
     if not args.no_cpu:
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(args.cpu_lmax, L, eps)
-        print("[bench] cpu backend ...", file=sys.stderr, flush=True)
-        cpu_be = cpu_backend_rate(args.cpu_lmax, L, eps)
+        cpu = cpu_baseline(L, eps)
+        print(f"[bench] cpu baseline: {cpu['sample']}", file=sys.stderr, flush=True)
+        cpu_be = cpu_backend_rate(L, eps)
+        print(f"[bench] cpu backend: {cpu_be['sample']}", file=sys.stderr, flush=True)
 
     res = {
         "metric": "spin-2 lensed maps/s at lmax 4095, eps 1e-7 (served vs plain); p50/p99 ms",

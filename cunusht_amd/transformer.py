@@ -10,9 +10,10 @@ Differences from the reference, all additive:
     CPU backend (cunusht/deflection/CPU_nufft_transformer.py:429-529); spin > 0 maps are (Q, U) rows.
   * one instance serves both directions (the reference plans one nuFFT type per instance).
   * `out` / `pointmap` / `alm` / `lenmap` arguments are written to when given.
-  * the pointing and point caches compare the caller's array with a device copy of the last one used
-    (the reference's full equality compare, GPU_nufft_transformer.py:417-420,470-473,550-556), so a
-    buffer refilled in place or a recycled temporary is never mistaken for the old points.
+  * the pointing and point caches compare a caller's array found at the address of the last one used
+    with a device copy of it (the reference's full equality compare, GPU_nufft_transformer.py:417-420,
+    470-473,550-556), so a buffer refilled in place or a recycled temporary is never mistaken for the old
+    points; an array at another address is taken as new without a compare.
 Errors: AssertionError for lmax/mmax/epsilon mismatches (as the reference), ValueError for unknown
 options, nusht NuShtError (RuntimeError) for failures inside the library.
 """
@@ -55,27 +56,28 @@ _GENERATION = itertools.count(1)
 
 
 class _Held:
-    """Device copy of the last array a cache was built from.  `same(t)` is the reference's full equality
-    compare (GPU_nufft_transformer.py:417-420,550-556): one device compare and one stream synchronization,
-    and nothing keyed on addresses, which the caching allocator recycles, or on version counters, which
-    writes through raw pointers do not bump."""
+    """Device copy of the last array a cache was built from.  `same(t)` holds when `t` sits at the address
+    of that array and has the same contents: the reference's full equality compare
+    (GPU_nufft_transformer.py:417-420,550-556; one device compare and one stream synchronization) for a
+    buffer passed again, which may have been refilled in place (through a raw pointer, which torch's version
+    counter does not see) or be a recycled temporary.  An array at another address is taken as new without
+    a compare (no synchronization): equal contents there cost one recomputation, never a stale result."""
 
     def __init__(self):
         self.copy = None
+        self.ptr = None
         self.gen = 0
 
     def same(self, t: torch.Tensor) -> bool:
         c = self.copy
-        return c is not None and c.shape == t.shape and c.dtype == t.dtype and bool(torch.equal(c, t))
+        return (c is not None and t.data_ptr() == self.ptr and c.shape == t.shape and c.dtype == t.dtype
+                and bool(torch.equal(c, t)))
 
     def hold(self, t: torch.Tensor) -> int:
         self.copy = t.detach().clone()
+        self.ptr = t.data_ptr()
         self.gen = next(_GENERATION)
         return self.gen
-
-    def clear(self):
-        self.copy = None
-        self.gen = 0
 
 
 class RingSHT:

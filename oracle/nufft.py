@@ -15,9 +15,20 @@ type-1/type-2 dot-product adjointness.
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
+from scipy import fft as sfft
 
 from . import clib
+
+
+def fft_workers() -> int:
+    """Threads for the oracle's FFTs (scipy.fft; the transform itself is the textbook DFT): the OpenMP
+    thread count when set, else the cores this process may use."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    n = len(os.sched_getaffinity(0))
+    return min(n, int(env)) if env.isdigit() and int(env) > 0 else n
 
 
 def es_params(eps: float):
@@ -51,7 +62,7 @@ def nufft2d2(F: np.ndarray, x1, x2, eps: float) -> np.ndarray:
     d2 = es_fourier(k2, nf2, w, beta)
     G = np.zeros((nf1, nf2), dtype=np.complex128)
     G[np.ix_(k1 % nf1, k2 % nf2)] = F / np.outer(d1, d2)
-    g = np.fft.ifft2(G) * (nf1 * nf2)
+    g = sfft.ifft2(G, workers=fft_workers()) * (nf1 * nf2)
     return clib.es_interp(g, x1, x2, w, beta)
 
 
@@ -63,7 +74,7 @@ def nufft2d1(c, x1, x2, shape, eps: float) -> np.ndarray:
     d1 = es_fourier(k1, nf1, w, beta)
     d2 = es_fourier(k2, nf2, w, beta)
     g = clib.es_spread((nf1, nf2), x1, x2, np.asarray(c, dtype=np.complex128), w, beta)
-    G = np.fft.fft2(g)
+    G = sfft.fft2(g, workers=fft_workers())
     return G[np.ix_(k1 % nf1, k2 % nf2)] / np.outer(d1, d2)
 
 

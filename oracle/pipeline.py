@@ -17,6 +17,7 @@ lmax, and by the dot-product adjointness of each pair (test/test_adjointOperator
 from __future__ import annotations
 
 import numpy as np
+from scipy import fft as sfft
 
 from . import clib, nufft
 from .almutil import almxfl
@@ -37,14 +38,14 @@ def phi_synth(G: np.ndarray, lmax: int, spin: int, nphi: int, phi0=None) -> np.n
         np.add.at(arr, (slice(None), (-neg) % nphi), np.conj(Gs[1:]).T)
     else:
         np.add.at(arr, (slice(None), ms % nphi), Gs.T)
-    out = np.fft.ifft(arr, axis=1) * nphi
+    out = sfft.ifft(arr, axis=1, workers=nufft.fft_workers()) * nphi
     return out.real if spin == 0 else out
 
 
 def phi_adjoint(M: np.ndarray, lmax: int, spin: int, phi0=None) -> np.ndarray:
     """Exact adjoint of phi_synth (per ring): G'_m = sum_k M_k e^{-i m phi_k}."""
     nring, nphi = M.shape
-    F = np.fft.fft(np.asarray(M, dtype=np.complex128), axis=1)
+    F = sfft.fft(np.asarray(M, dtype=np.complex128), axis=1, workers=nufft.fft_workers())
     ms = np.arange(lmax + 1) if spin == 0 else np.arange(-lmax, lmax + 1)
     G = F[:, ms % nphi].T
     if phi0 is not None:
@@ -85,7 +86,7 @@ class NuSHT:
         G = clib.sht_synth_rings(alm, L, spin, self.thetas)
         cmap = phi_synth(G, L, spin, self.nphi)
         d = self._doubling(cmap, spin)
-        F = np.fft.fft2(d) / d.size
+        F = sfft.fft2(d, workers=nufft.fft_workers()) / d.size
         c = nufft.nufft2d2(F, theta, phi, eps)
         return c.real if spin == 0 else c
 
@@ -93,7 +94,7 @@ class NuSHT:
         L = self.lmax
         shape = (2 * self.ntheta - 2, self.nphi)
         F = nufft.nufft2d1(np.asarray(values, dtype=np.complex128), theta, phi, shape, eps)
-        d = np.fft.ifft2(F)
+        d = sfft.ifft2(F, workers=nufft.fft_workers())
         cmap = self._adjoint_doubling(d, spin)
         if spin == 0:
             cmap = cmap.real
@@ -112,11 +113,12 @@ def dlm2pointing(dlm_scaled, lmax: int, rings: Rings, with_gamma: bool = True):
     th, ph = rings.pixel_angles()
     dt = np.empty(th.size)
     dp = np.empty(th.size)
-    for i in range(rings.theta.size):
-        v = phi_synth(G[:, i:i + 1], lmax, 1, int(rings.nph[i]), phi0=np.array([rings.phi0[i]]))[0]
-        s = slice(int(rings.ofs[i]), int(rings.ofs[i] + rings.nph[i]))
-        dt[s] = v.real
-        dp[s] = v.imag
+    for n in np.unique(rings.nph):  # rings of one length at once
+        sel = np.flatnonzero(rings.nph == n)
+        v = phi_synth(G[:, sel], lmax, 1, int(n), phi0=rings.phi0[sel])
+        pix = (rings.ofs[sel][:, None] + np.arange(n)[None, :]).reshape(-1)
+        dt[pix] = v.real.reshape(-1)
+        dp[pix] = v.imag.reshape(-1)
     tp, pp = deflected_angles(th, ph, dt, dp)
     g = gamma_fn(th, dt, dp) if with_gamma else None
     return tp, pp, g, (dt, dp)

@@ -62,7 +62,7 @@ def test_nusht_fresh_host_loc_every_call():
     got = T.nusht2d1(L, L, x, loc).cpu().numpy()
     ref = osht.adjoint_direct(x, L, 0, loc[:, 0], loc[:, 1])
     assert _relmax(got, ref) < 1e-9
-    # the same values again, in a new host array: still correct (and no new binning is needed)
+    # the same values again, in a new host array: still correct
     got2 = T.nusht2d1(L, L, x, loc.copy()).cpu().numpy()
     assert _relmax(got2, ref) < 1e-9
 
