@@ -84,11 +84,11 @@ def stage_work(stage: str, info: dict, lmax: int, npix: int, maps_per_launch: in
 
 def executed_flops(stage: str, info: dict, maps_per_launch: int = 1, rinfo: dict | None = None):
     """Flops the Legendre kernels actually run: executed (l, ring pair) steps x (7 flops of
-    recursion + 16 of contraction per map)."""
+    recursion + 16 of contraction per E + B map, 8 for the E-only gradient of the pointing)."""
     if stage == "legendre_synth_cc_s2":
         return info["legendre_pair_steps"] * (7 + 16 * maps_per_launch)
     if stage == "legendre_synth_ring_s1" and rinfo is not None:
-        return rinfo["legendre_pair_steps"] * (7 + 16)
+        return rinfo["legendre_pair_steps"] * (7 + 8)
     return None
 
 

@@ -47,6 +47,7 @@ struct LegendreTables {
   int chunk = 2;          // maps per synthesis launch (VALU kernels)
   bool mfma = true;       // batches of >= 3 maps on the matrix cores (NUSHT_LEGENDRE_MFMA=0 turns it off)
   int variant = -1;       // synthesis kernel variant (-1: default); set by legendre_autotune
+  int variant_grad = -1;  // the same for the E-only (gradient) contraction of spin-s tables
   float tuned_ms = 0.f;
 
   void build(int lmax, int spin, const std::vector<double>& ring_theta, hipStream_t st);
@@ -71,7 +72,8 @@ void legendre_synth(const LegendreTables& t, int nmaps, int ncomp, const double2
 // Number of synthesis kernel variants.
 int legendre_variant_count();
 
-// Times every synthesis variant on the given buffers (contents are overwritten) and keeps the fastest
+// Times every synthesis variant on the given buffers (contents are overwritten) and keeps the fastest for
+// this ncomp (spin-s tables keep one variant for E + B and one for E alone)
 // (or takes the one named by the environment variable NUSHT_LEGENDRE_VARIANT).
 // (Plan creation only: allocates a scratch alm set and synchronizes.)
 void legendre_autotune(LegendreTables& t, int ncomp, double2* coef_ws, double2* G, long sm, long sr, hipStream_t st);

@@ -168,7 +168,8 @@ __global__ __launch_bounds__(64) void k_advance(int lmax, int s, int nslot, int 
 // ------------------------------------------------------------------ synthesis prep
 // coef layout per map b, (m, l):  spin 0: coef[(mofs[m]+l-l0)*NB + b] = a_lm c_l fl_l
 //                                 spin s: coef[((mofs[m]+l-l0)*NB + b)*2 + {0,1}] = (W', Wt')
-//   W' = -(-1)^s c_l (E + iB),  Wt' = -c_l (conj E + i conj B).
+//                                 spin s, E only (ncomp 1): coef[(mofs[m]+l-l0)*NB + b] = W'
+//   W' = -(-1)^s c_l (E + iB),  Wt' = -c_l (conj E + i conj B);  with B = 0, Wt' = (-1)^s conj(W').
 __global__ void k_prep(int lmax, int s, int nmaps, int nbpad, int ncomp, const double2* __restrict__ alm, long stride_map,
                        const double* __restrict__ fl, const long* __restrict__ mofs,
                        const double* __restrict__ cnorm, double2* __restrict__ coef) {
@@ -192,6 +193,8 @@ __global__ void k_prep(int lmax, int s, int nmaps, int nbpad, int ncomp, const d
       }
       if (s == 0) {
         coef[q * nbpad + b] = double2{e.x * c, e.y * c};
+      } else if (ncomp == 1) {
+        coef[q * nbpad + b] = double2{sg * c * e.x, sg * c * e.y};
       } else {
         double2 w{e.x - bb.y, e.y + bb.x};
         double2 wt{e.x + bb.y, -e.y + bb.x};
@@ -235,10 +238,15 @@ struct SynthArgs {
 // entries so a chunk never reads past the end.
 constexpr int kSynCH = 8;  // largest chunk of any variant (table padding)
 
-template <bool SP, int NB, int CH, int WPB>
+// GR (spin s, E only): the coefficient stream holds W' alone.  Since Wt' = (-1)^s conj(W') and the
+// recursion values are real, the -m sums are conjugates of the +m ones: the kernel accumulates
+// sum W' mu_u and sum W' mu_v split by the parity of l - l0 (half the contraction of the E + B case) and
+// forms the four ring coefficients at the end.
+template <bool SP, int NB, int CH, int WPB, bool GR = false>
 __global__ __launch_bounds__(64 * WPB) void k_synth(SynthArgs a) {
+  static_assert(SP || !GR, "the gradient-only contraction is a spin-s case");
   constexpr int R = 1;
-  constexpr int K = SP ? 2 * NB : NB;
+  constexpr int K = (SP && !GR) ? 2 * NB : NB;
   // Blocks are dealt to the XCDs round-robin (block b runs on XCD b % 8).  XCD x takes
   // m = x, x + 8, x + 16, ... with all blocks of one m in a row, so each m's coefficient stream
   // is read through one XCD's L2, and every XCD still starts with its longest m.  The WPB waves
@@ -318,6 +326,11 @@ __global__ __launch_bounds__(64 * WPB) void k_synth(SynthArgs a) {
         for (int b = 0; b < NB; ++b) {
           if (!SP) {
             if (odd) fma2(A1[r][b], bf.cw[i][b], wu); else fma2(A0[r][b], bf.cw[i][b], wu);
+          } else if (GR) {
+            // MS0 / MS1 hold sum W' mu_u by parity here
+            const double2 w = bf.cw[i][b];
+            if (odd) { fma2(MS1[r][b], w, wu); fma2(A1[r][b], w, wv); }
+            else { fma2(MS0[r][b], w, wu); fma2(A0[r][b], w, wv); }
           } else {
             const double2 w = bf.cw[i][2 * b], wt = bf.cw[i][2 * b + 1];
             fma2(PN[r][b], w, wu);
@@ -387,6 +400,22 @@ __global__ __launch_bounds__(64 * WPB) void k_synth(SynthArgs a) {
         double2 gs = double2{sg0 * (A0[r][b].x - A1[r][b].x), sg0 * (A0[r][b].y - A1[r][b].y)};
         Gb[(long)m * a.sm + (long)rn * a.sr] = gn;
         if (rs >= 0) Gb[(long)m * a.sm + (long)rs * a.sr] = gs;
+      } else if (GR) {
+        // north +m: sum W' mu_u; north -m: (-1)^s conj(sum W' mu_v); south +m: sg0 (A0 - A1);
+        // south -m: sg0 (-1)^s conj(even - odd of sum W' mu_u)
+        const double ss = (a.s & 1) ? -1.0 : 1.0;
+        const double2 pn = cadd(MS0[r][b], MS1[r][b]);
+        const double2 av = cadd(A0[r][b], A1[r][b]);
+        const double2 pS = double2{sg0 * (A0[r][b].x - A1[r][b].x), sg0 * (A0[r][b].y - A1[r][b].y)};
+        const double2 mN = double2{ss * av.x, -ss * av.y};
+        const double2 mS = double2{sg0 * ss * (MS0[r][b].x - MS1[r][b].x), -sg0 * ss * (MS0[r][b].y - MS1[r][b].y)};
+        const long rowp = (long)(L + m) * a.sm, rowm = (long)(L - m) * a.sm;
+        Gb[rowp + (long)rn * a.sr] = pn;
+        if (rs >= 0) Gb[rowp + (long)rs * a.sr] = pS;
+        if (m > 0) {
+          Gb[rowm + (long)rn * a.sr] = mN;
+          if (rs >= 0) Gb[rowm + (long)rs * a.sr] = mS;
+        }
       } else {
         double2 pS = double2{sg0 * (A0[r][b].x - A1[r][b].x), sg0 * (A0[r][b].y - A1[r][b].y)};
         double2 mS = double2{sg0 * (MS0[r][b].x - MS1[r][b].x), sg0 * (MS0[r][b].y - MS1[r][b].y)};
@@ -1073,7 +1102,7 @@ struct SynVariant {
 static const SynVariant kSynVariants[] = {{2, 1}, {2, 4}, {4, 4}, {2, 8}};
 constexpr int kNumSynVariants = 4;
 
-template <bool SP, int NB>
+template <bool SP, int NB, bool GR = false>
 static void launch_synth(const SynthArgs& a, int variant, hipStream_t st) {
   // Odd number of blocks per m: blocks are handed to XCDs and shader engines round-robin, and
   // with a power-of-two count every engine would see the same slot groups for every m, so the
@@ -1085,10 +1114,10 @@ static void launch_synth(const SynthArgs& a, int variant, hipStream_t st) {
   b.nsb = nsb;
   const dim3 blk(64 * wpb);
   switch (variant) {
-    case 0: hipLaunchKernelGGL((k_synth<SP, NB, 2, 1>), dim3(nblocks), blk, 0, st, b); break;
-    case 1: hipLaunchKernelGGL((k_synth<SP, NB, 2, 4>), dim3(nblocks), blk, 0, st, b); break;
-    case 2: hipLaunchKernelGGL((k_synth<SP, NB, 4, 4>), dim3(nblocks), blk, 0, st, b); break;
-    default: hipLaunchKernelGGL((k_synth<SP, NB, 2, 8>), dim3(nblocks), blk, 0, st, b); break;
+    case 0: hipLaunchKernelGGL((k_synth<SP, NB, 2, 1, GR>), dim3(nblocks), blk, 0, st, b); break;
+    case 1: hipLaunchKernelGGL((k_synth<SP, NB, 2, 4, GR>), dim3(nblocks), blk, 0, st, b); break;
+    case 2: hipLaunchKernelGGL((k_synth<SP, NB, 4, 4, GR>), dim3(nblocks), blk, 0, st, b); break;
+    default: hipLaunchKernelGGL((k_synth<SP, NB, 2, 8, GR>), dim3(nblocks), blk, 0, st, b); break;
   }
   NUSHT_LAUNCH_CHECK();
 }
@@ -1185,11 +1214,21 @@ static void legendre_synth_chunk(const LegendreTables& t, int nmaps, int ncomp, 
   a.coef = coef_ws;
   a.G = G; a.sm = sm; a.sr = sr; a.smap = smap;
   const bool sp = t.spin > 0;
-  const int var = (t.variant >= 0 && t.variant < kNumSynVariants) ? t.variant : 0;
+  const bool gr = sp && ncomp == 1;  // E only: the gradient-only contraction
+  const int tv = gr ? t.variant_grad : t.variant;
+  const int var = (tv >= 0 && tv < kNumSynVariants) ? tv : 0;
   StageScope sc(t.n_synth.c_str(), st);
   switch (nbpad) {  // the kernel's NB must equal the coefficient stride written by k_prep
-    case 1: sp ? launch_synth<true, 1>(a, var, st) : launch_synth<false, 1>(a, var, st); break;
-    case 2: sp ? launch_synth<true, 2>(a, var, st) : launch_synth<false, 2>(a, var, st); break;
+    case 1:
+      if (gr) launch_synth<true, 1, true>(a, var, st);
+      else if (sp) launch_synth<true, 1>(a, var, st);
+      else launch_synth<false, 1>(a, var, st);
+      break;
+    case 2:
+      if (gr) launch_synth<true, 2, true>(a, var, st);
+      else if (sp) launch_synth<true, 2>(a, var, st);
+      else launch_synth<false, 2>(a, var, st);
+      break;
     default: throw Error{kErrUnsupported, "legendre_synth: at most 2 maps per launch"};
   }
 }
@@ -1201,10 +1240,12 @@ void legendre_autotune(LegendreTables& t, int ncomp, double2* coef_ws, double2* 
   if (const char* fv = getenv("NUSHT_LEGENDRE_VARIANT"); fv && fv[0]) {
     const int v = atoi(fv);
     NUSHT_REQUIRE(v >= 0 && v < kNumSynVariants, kErrArg, "NUSHT_LEGENDRE_VARIANT out of range");
-    t.variant = v;
+    t.variant = t.variant_grad = v;
     t.tuned_ms = 0.f;
     return;
   }
+  const bool gr = t.spin > 0 && ncomp == 1;
+  int& slot = gr ? t.variant_grad : t.variant;
   // one measurement per (lmax, spin, ring pairs) and process
   static std::mutex mu;
   static std::map<std::tuple<int, int, int, int>, std::pair<int, float>> cache;
@@ -1212,7 +1253,11 @@ void legendre_autotune(LegendreTables& t, int ncomp, double2* coef_ws, double2* 
   {
     std::lock_guard<std::mutex> g(mu);
     auto it = cache.find(key);
-    if (it != cache.end()) { t.variant = it->second.first; t.tuned_ms = it->second.second; return; }
+    if (it != cache.end()) {
+      slot = it->second.first;
+      if (!gr) t.tuned_ms = it->second.second;
+      return;
+    }
   }
   DevBuf dummy(sizeof(double2) * (size_t)(t.lmax + 1) * (t.lmax + 2) / 2 * ncomp);
   NUSHT_HIP(hipMemsetAsync(dummy.get(), 0, dummy.bytes(), st));
@@ -1225,7 +1270,7 @@ void legendre_autotune(LegendreTables& t, int ncomp, double2* coef_ws, double2* 
   const char* env = getenv("NUSHT_VERBOSE");
   const bool verbose = env && env[0] && env[0] != '0';
   for (int v = 0; v < kNumSynVariants; ++v) {
-    t.variant = v;
+    slot = v;
     legendre_synth(t, 1, ncomp, alm, 0, nullptr, coef_ws, G, sm, sr, 0, st);  // warm-up
     NUSHT_HIP(hipEventRecord(e0, st));
     for (int k = 0; k < 2; ++k) legendre_synth(t, 1, ncomp, alm, 0, nullptr, coef_ws, G, sm, sr, 0, st);
@@ -1238,12 +1283,12 @@ void legendre_autotune(LegendreTables& t, int ncomp, double2* coef_ws, double2* 
               t.tag.c_str(), t.lmax, t.spin, t.nslot, v, kSynVariants[v].CH, kSynVariants[v].WPB, ms / 2);
     if (ms < best) { best = ms; bestv = v; }
   }
-  t.variant = bestv;
-  t.tuned_ms = best / 2;
+  slot = bestv;
+  if (!gr) t.tuned_ms = best / 2;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   std::lock_guard<std::mutex> g(mu);
-  cache[key] = {t.variant, t.tuned_ms};
+  cache[key] = {bestv, best / 2};
 }
 
 void legendre_adjoint(const LegendreTables& t, int ncomp, const double2* G, long sm, long sr, double2* alm,

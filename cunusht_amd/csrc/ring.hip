@@ -185,6 +185,7 @@ RingPlan::RingPlan(int lmax, int spin, const std::vector<double>& theta, const s
   hipLaunchKernelGGL(k_sqrt_ll, dim3(ceil_div(lmax + 1, 256)), dim3(256), 0, st, lmax, sqll_.as<double>());
   NUSHT_LAUNCH_CHECK();
   legendre_autotune(tab_, spin ? 2 : 1, coef_.as<double2>(), G_.as<double2>(), gstride_, 1, st);
+  if (spin) legendre_autotune(tab_, 1, coef_.as<double2>(), G_.as<double2>(), gstride_, 1, st);  // E only
 }
 
 size_t RingPlan::bytes() const {

