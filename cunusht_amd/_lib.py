@@ -50,7 +50,9 @@ class Job(ctypes.Structure):
 class PlanInfo(ctypes.Structure):
     _fields_ = [("ntheta", _i), ("ntheta_dfs", _i), ("nfine", _i), ("nrow_fine", _i), ("kernel_width", _i),
                 ("single_precision", _i), ("sigma", _d), ("beta", _d), ("workspace_bytes", _i64),
-                ("legendre_pair_steps", _i64), ("legendre_variant", _i), ("legendre_ms", ctypes.c_float)]
+                ("legendre_pair_steps", _i64), ("legendre_variant", _i), ("legendre_ms", ctypes.c_float),
+                ("nufft_tile_rows", _i), ("nufft_tile_cols", _i), ("nufft_candidates", _i),
+                ("nufft_type2_ms", ctypes.c_float), ("nufft_type1_ms", ctypes.c_float)]
 
 
 class RingInfo(ctypes.Structure):

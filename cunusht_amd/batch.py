@@ -29,11 +29,12 @@ class LensingPlan:
     on different LensingPlans concurrently."""
 
     def __init__(self, lmax: int, spin: int, epsilon: float, geometry=None, ncomp: int | None = None,
-                 max_maps: int = 1):
+                 max_maps: int = 1, sigma: float = 0.0):
+        """sigma: nuFFT oversampling, 0 automatic, < 0 chosen by measurement at plan creation."""
         self.geom: Geom = geometry if isinstance(geometry, Geom) else get_geom(geometry or ("gl", {"lmax": lmax}))
         self.lmax, self.spin, self.epsilon = lmax, spin, float(epsilon)
         self.ncomp = (1 if spin == 0 else 2) if ncomp is None else ncomp
-        self.nuplan = NuPlan(lmax, spin, self.ncomp, self.epsilon, self.geom.npix(), 0.0, max_maps)
+        self.nuplan = NuPlan(lmax, spin, self.ncomp, self.epsilon, self.geom.npix(), sigma, max_maps)
         self.ring = RingSHT(self.geom, lmax, 1)
 
     def npix(self) -> int:

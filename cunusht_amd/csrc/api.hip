@@ -28,6 +28,7 @@ struct nusht_plan_s {
   DevBuf coef, G, ptg;
   int nmrows = 0;
   int max_maps = kMaxMaps;  // maps per batched synthesis the G buffer holds
+  NufftTune tune;           // plan-time nuFFT measurement (candidates = 0: not tuned)
   static constexpr int kMaxMaps = 8;
   size_t bytes() const { return tab.bytes() + nu->bytes() + coef.bytes() + G.bytes() + ptg.bytes(); }
 };
@@ -251,18 +252,26 @@ int nusht_plan_create(const nusht_plan_desc* d, nusht_stream_t s, nusht_plan_t* 
     p->spin = d->spin;
     p->ncomp = d->ncomp;
     p->npts_max = std::max<long>(d->npts_max, 1);
-    p->np = NufftParams::choose(d->lmax, d->spin, d->epsilon, d->sigma);
+    const char* at = getenv("NUSHT_NUFFT_AUTOTUNE");
+    const bool tune = d->sigma < 0 || (d->sigma == 0 && at && at[0] == '1');
+    p->np = NufftParams::choose(d->lmax, d->spin, d->epsilon, d->sigma > 0 ? d->sigma : 0.0);
     std::vector<double> th(p->np.ntheta);
     for (int j = 0; j < p->np.ntheta; ++j) th[j] = M_PI * j / (p->np.ntheta - 1);
     p->tab.tag = "cc";
     p->tab.build(d->lmax, d->spin, th, S(s));
-    p->nu = std::make_unique<NufftPlan>(p->np, p->npts_max, S(s));
     NUSHT_REQUIRE(d->max_maps >= 0 && d->max_maps <= nusht_plan_s::kMaxMaps, kErrArg, "max_maps must be in [0, 8]");
     p->max_maps = d->max_maps ? d->max_maps : nusht_plan_s::kMaxMaps;
     p->nmrows = d->spin ? 2 * d->lmax + 1 : d->lmax + 1;
     p->coef.alloc(legendre_coef_bytes(p->tab, p->max_maps));
     p->G.alloc(sizeof(double2) * (size_t)p->nmrows * p->np.ntheta * p->max_maps);
     p->ptg.alloc(sizeof(double) * 3 * (size_t)p->npts_max);
+    if (tune) {  // the CC grid (ntheta) does not depend on sigma: the Legendre tables stay
+      p->nu = nufft_autotune(d->lmax, d->spin, d->epsilon, p->npts_max, p->G.as<double2>(), p->np.ntheta, S(s),
+                             p->tune);
+      p->np = p->nu->params();
+    } else {
+      p->nu = std::make_unique<NufftPlan>(p->np, p->npts_max, S(s));
+    }
     legendre_autotune(p->tab, p->ncomp, p->coef.as<double2>(), p->G.as<double2>(), p->np.ntheta, 1, S(s));
     NUSHT_HIP(hipStreamSynchronize(S(s)));
     *out = p.release();
@@ -289,6 +298,11 @@ int nusht_plan_get_info(nusht_plan_t p, nusht_plan_info* info) {
     info->legendre_pair_steps = p->tab.pair_steps;
     info->legendre_variant = p->tab.variant;
     info->legendre_ms = p->tab.tuned_ms;
+    info->nufft_tile_rows = p->nu->tile_rows();
+    info->nufft_tile_cols = p->nu->tile_cols();
+    info->nufft_candidates = p->tune.candidates;
+    info->nufft_type2_ms = p->tune.type2_ms;
+    info->nufft_type1_ms = p->tune.type1_ms;
   });
 }
 

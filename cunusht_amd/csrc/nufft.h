@@ -17,6 +17,8 @@
 // points with theta in [0, pi] can touch (about nf/2 + w of nf) are ever formed.
 #pragma once
 
+#include <memory>
+
 #include "common.h"
 #include "fft_engine.h"
 
@@ -45,7 +47,11 @@ struct NufftParams {
 // Precomputed tables and per-plan buffers.
 class NufftPlan {
  public:
-  NufftPlan(const NufftParams& p, long npts_max, hipStream_t st);
+  // tile_rows: rows of the register row-window spreading tiles (0: 16, or NUSHT_SPREAD_TR)
+  NufftPlan(const NufftParams& p, long npts_max, hipStream_t st, int tile_rows = 0);
+  int tile_rows() const { return tile_r_; }
+  int tile_cols() const { return tile_c_; }
+  bool rows_spread() const { return rows_spread_; }
   const NufftParams& params() const { return p_; }
 
   // Sort points into fine-grid tiles (theta, phi in radians; theta in [0, pi]).
@@ -76,6 +82,8 @@ class NufftPlan {
   long npts_max_ = 0;
   long npts_ = 0;
   int tile_r_ = 32, tile_c_ = 32;  // spreading tile (fine cells)
+  bool rows_spread_ = false;       // register row-window spreading (8 <= w <= 16)
+  int rows_cap_ = 0;               // its LDS point records per pass
   int ntile_r_ = 0, ntile_c_ = 0;
   // tables
   DevBuf deconv_;                  // double[lmax+1] : 1 / psi_hat(k)
@@ -94,5 +102,20 @@ class NufftPlan {
   DevBuf scan_tmp_;                // hipcub scan temp storage
   size_t scan_tmp_bytes_ = 0;
 };
+
+// Plan-time choice of the fine-grid oversampling sigma and the spreading tile height by measurement: every
+// candidate (sigma in {1.25, 1.5, 2} and the default rule, as far as the fine grid fits the in-LDS
+// transforms; tile heights 8, 16, 32 where the register row-window spreading applies) is built and timed on
+// npts_max synthetic points (theta = acos U(-1, 1), phi = U(0, 2 pi)) for one type-2 and one type-1
+// transform from / into G (rows m with stride g_stride_m; contents are overwritten); the plan with the
+// least type-2 + type-1 time is returned.  Results are cached per (device, lmax, spin, eps, npts_max).
+struct NufftTune {
+  double sigma = 0;
+  int tile_rows = 0;
+  float type2_ms = 0, type1_ms = 0;
+  int candidates = 0;
+};
+std::unique_ptr<NufftPlan> nufft_autotune(int lmax, int spin, double eps, long npts_max, double2* G, long g_stride_m,
+                                          hipStream_t st, NufftTune& tune);
 
 }  // namespace nusht

@@ -14,6 +14,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
+#include <tuple>
+#include <vector>
 
 namespace nusht {
 namespace {
@@ -579,6 +583,218 @@ __global__ __launch_bounds__(256) void k_spread_w(SpreadArgs<T> a) {
 // Widths with a fully unrolled kernel; wider kernels use the runtime-width versions.
 constexpr int kMaxUnrolledW = 16;
 
+// ------------------------------------------------------------------ spreading by register row windows
+// Widths kRowsMinW..kRowsMaxW.  A tile is tr rows x 16 SW columns of first cells (i0, j0), SW = 17 - W:
+// 16-lane group g owns the 16 fine columns [g SW, g SW + 16) of the tile (+ halo), which hold the whole
+// W-column footprint of every point whose first column lies in the group's slice [g SW, g SW + SW).  The
+// points of the tile are counting-sorted in LDS by (slice, first row); each group then walks the rows
+// r = 0, 1, ... of its slice, adding every point of row r into W rows of register accumulators (lane j:
+// column g SW + j; kernel row weights by DPP row broadcasts, the column weight shuffled in by the point's
+// column offset).  Row r receives nothing after the points of row r, so its accumulator is added to the
+// LDS tile once (16 lanes, one ds_add_f64 pair) and reused as row r + W: a rotating window whose slot
+// index r mod W is wave-uniform (the four groups of a wave take slices w, w + 4, w + 8, w + 12 and step
+// through the rows together), so every register index is a compile-time constant of the W-fold unrolled
+// row loop.  LDS atomics: two per fine cell and covering group instead of 2 W^2 per point.  Tiles with more
+// points than the LDS record buffer holds are spread in several passes.
+constexpr int kRowsMinW = 8;
+constexpr int kRowsMaxW = 16;
+constexpr int kRowsGroups = 16;  // 16-lane groups per 256-thread workgroup (= slices per tile)
+
+struct RowsLayout {
+  int tr, tc, aw, cap, deg, nkey;
+  size_t acc_off, val_off, uu_off, off_off, cnt_off, coef_off, bytes;
+};
+
+template <typename T>
+__host__ __device__ inline RowsLayout rows_layout(int w, int tr, int deg, int cap) {
+  RowsLayout L;
+  const int sw = 17 - w;
+  L.tr = tr;
+  L.tc = kRowsGroups * sw;
+  L.aw = L.tc + w - 1;
+  L.cap = cap;
+  L.deg = deg;
+  L.nkey = kRowsGroups * tr;
+  size_t o = 0;
+  L.acc_off = o; o += sizeof(double2) * (size_t)(tr + w - 1) * L.aw;
+  L.val_off = o; o += sizeof(double2) * (size_t)cap;
+  L.uu_off = o;  o += sizeof(c2_t<T>) * (size_t)cap;
+  L.off_off = o; o += (size_t)cap;
+  o = (o + 15) & ~(size_t)15;
+  L.cnt_off = o; o += sizeof(int) * (size_t)(2 * L.nkey + 1);
+  o = (o + 15) & ~(size_t)15;
+  L.coef_off = o; o += sizeof(T) * 16 * (size_t)(deg + 1);
+  L.bytes = o;
+  return L;
+}
+
+template <typename T, int W>
+__global__ __launch_bounds__(256) void k_spread_rows(SpreadArgs<T> a, RowsLayout lay) {
+  static_assert(W >= kRowsMinW && W <= kRowsMaxW, "register row windows for 8 <= W <= 16");
+  constexpr int SW = 17 - W;
+  using T2 = c2_t<T>;
+  extern __shared__ unsigned char smem_raw[];
+  double* acc = reinterpret_cast<double*>(smem_raw + lay.acc_off);  // interleaved re, im; (tr + W - 1) x aw
+  double2* val = reinterpret_cast<double2*>(smem_raw + lay.val_off);
+  T2* uu = reinterpret_cast<T2*>(smem_raw + lay.uu_off);
+  unsigned char* offs = smem_raw + lay.off_off;
+  int* cnt = reinterpret_cast<int*>(smem_raw + lay.cnt_off);  // [nkey]: counts, then cursors
+  int* start = cnt + lay.nkey;                                 // [nkey + 1]
+  T* coef = reinterpret_cast<T*>(smem_raw + lay.coef_off);     // [(deg + 1) x 16], cells >= W zero
+  const int tb = blockIdx.x;
+  const int s0 = a.tile_start[tb], s1 = a.tile_start[tb + 1];
+  if (s0 == s1) return;
+  const int tr = lay.tr, aw = lay.aw, nkey = lay.nkey, deg = lay.deg;
+  const int nacc = (tr + W - 1) * aw;
+  for (int i = threadIdx.x; i < 2 * nacc; i += blockDim.x) acc[i] = 0.0;
+  for (int i = threadIdx.x; i < 16 * (deg + 1); i += blockDim.x) {
+    const int k = i >> 4, j = i & 15;
+    coef[i] = j < W ? a.kpoly[(long)k * W + j] : T(0);
+  }
+  const int tile_r = tb / a.ntc, tile_c = tb % a.ntc;
+  const int r0 = tile_r * a.tr, c0 = tile_c * a.tc;
+  const double scale = a.nf / (2.0 * M_PI);
+  const double hw = 0.5 * W;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int l16 = lane & 15, gsub = lane >> 4;
+  const int slice = wave + 4 * gsub;  // the four groups of a wave own non-overlapping windows
+  const int gbase = threadIdx.x & ~15;  // first thread of this lane's group
+  // first cells of a binned point, relative to the tile; lr / lc outside the tile never happen
+  auto cells = [&](int p, int& lr, int& lc, double& ut, double& up, int& i0, int& j0u) {
+    const double2 tp = a.pts[p];
+    ut = tp.x * scale;
+    up = tp.y * scale;
+    up -= a.nf * floor(up / a.nf);
+    i0 = (int)ceil(ut - hw);
+    j0u = (int)ceil(up - hw);
+    const int j0 = j0u < 0 ? j0u + a.nf : j0u;
+    lr = i0 - a.row_lo - r0;
+    lc = j0 - c0;
+  };
+  for (int cb = s0; cb < s1; cb += lay.cap) {
+    const int n = min(lay.cap, s1 - cb);
+    for (int i = threadIdx.x; i < nkey; i += blockDim.x) cnt[i] = 0;
+    __syncthreads();
+    // pass 1: counts per (slice, row)
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+      int lr, lc, i0, j0u;
+      double ut, up;
+      cells(cb + q, lr, lc, ut, up, i0, j0u);
+      if (lr >= 0 && lr < tr && lc >= 0 && lc < lay.tc) atomicAdd(&cnt[(lc / SW) * tr + lr], 1);
+    }
+    __syncthreads();
+    // exclusive scan (one wave, nkey <= 16 x 32)
+    if (wave == 0) {
+      const int per = (nkey + 63) / 64;
+      int sum = 0;
+      for (int i = 0; i < per; ++i) {
+        const int k = lane * per + i;
+        sum += k < nkey ? cnt[k] : 0;
+      }
+      int incl = sum;
+      for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += t;
+      }
+      int run = incl - sum;
+      for (int i = 0; i < per; ++i) {
+        const int k = lane * per + i;
+        if (k < nkey) {
+          start[k] = run;
+          run += cnt[k];
+        }
+      }
+      if (lane == 63) start[nkey] = incl;
+    }
+    __syncthreads();
+    // pass 2: records in sorted order (cnt counts down as the cursor)
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+      const int p = cb + q;
+      int lr, lc, i0, j0u;
+      double ut, up;
+      cells(p, lr, lc, ut, up, i0, j0u);
+      if (!(lr >= 0 && lr < tr && lc >= 0 && lc < lay.tc)) continue;
+      const int key = (lc / SW) * tr + lr;
+      const int pos = start[key] + atomicSub(&cnt[key], 1) - 1;
+      const int o = a.idx[p];
+      double vr = a.in_re[o];
+      double vi = (a.spin && a.in_im) ? a.in_im[o] : 0.0;
+      if (a.spin && a.gamma) {
+        double sn, cs;
+        sincos(-a.spin * a.gamma[o], &sn, &cs);
+        const double xr = vr * cs - vi * sn, xi = vr * sn + vi * cs;
+        vr = xr;
+        vi = xi;
+      }
+      val[pos] = double2{vr, vi};
+      uu[pos] = T2{(T)((i0 - ut) + 0.5 * W - 0.5), (T)((j0u - up) + 0.5 * W - 0.5)};
+      offs[pos] = (unsigned char)(lc % SW);
+    }
+    __syncthreads();
+    // row walk: register window of W rows x (re, im), slot of row r = r mod W
+    double ar[W], ai[W];
+#pragma unroll
+    for (int i = 0; i < W; ++i) ar[i] = ai[i] = 0.0;
+    const int kbase = slice * tr;
+    double* acol = acc + 2 * (slice * SW + l16);
+    for (int rb = 0; rb < tr + W - 1; rb += W) {
+      static_for<0, W>([&](auto kk) {
+        constexpr int K = decltype(kk)::value;
+        const int r = rb + K;  // wave-uniform
+        if (r < tr) {
+          const int b = start[kbase + r], e = start[kbase + r + 1];
+          for (int q = b; __any(q < e); ++q) {
+            if (q < e) {  // uniform over the 16-lane group
+              const double2 v = val[q];
+              const T2 u = uu[q];
+              const int off = offs[q];
+              const T* c = coef + deg * 16 + l16;
+              T kt = *c, kp = kt;
+              for (int k = deg - 1; k >= 0; --k) {
+                c -= 16;
+                kt = fma(kt, u.x, *c);
+                kp = fma(kp, u.y, *c);
+              }
+              const int src = l16 - off;  // footprint cell of this lane's column
+              const T kpc_ = __shfl(kp, gbase + (src & 15), 64);
+              const double kpc = (src >= 0 && src < W) ? (double)kpc_ : 0.0;
+              const double cr = v.x * kpc, ci = v.y * kpc;
+              static_for<0, W>([&](auto ic) {
+                constexpr int i = decltype(ic)::value;
+                constexpr int slot = (K + i) % W;
+                const double kti = (double)row_bcast<i>(kt);
+                ar[slot] = fma(cr, kti, ar[slot]);
+                ai[slot] = fma(ci, kti, ai[slot]);
+              });
+            }
+          }
+        }
+        if (r < tr + W - 1) {  // row r is complete: add it to the tile and reuse the slot for row r + W
+          if (ar[K] != 0.0 || ai[K] != 0.0) {
+            atomicAdd(&acol[2 * r * aw], ar[K]);
+            atomicAdd(&acol[2 * r * aw + 1], ai[K]);
+          }
+          ar[K] = 0.0;
+          ai[K] = 0.0;
+        }
+      });
+    }
+    __syncthreads();  // records are rewritten by the next pass
+  }
+  // the tile (+ halo) onto the fine grid
+  for (int i = threadIdx.x; i < nacc; i += blockDim.x) {
+    const int rr = i / aw, cc = i - rr * aw;
+    const int grow = r0 + rr;
+    if (grow >= a.nrow) continue;
+    const T vr = (T)acc[2 * i], vi = (T)acc[2 * i + 1];
+    if (vr == T(0) && vi == T(0)) continue;
+    const int gc = (c0 + cc) % a.nf;  // a tile can be wider than a small grid
+    T* g = reinterpret_cast<T*>(a.grid + (long)grow * a.nf + gc);
+    atomicAdd(&g[0], vr);
+    atomicAdd(&g[1], vi);
+  }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void k_spread(SpreadArgs<T> a) {
   extern __shared__ unsigned char smem_raw[];
@@ -794,6 +1010,21 @@ void launch_spread(int w, dim3 grid, size_t lds, hipStream_t st, const SpreadArg
   }
 }
 
+template <typename T, int W>
+void launch_spread_rows(int w, dim3 grid, hipStream_t st, const SpreadArgs<T>& a, const RowsLayout& lay) {
+  if constexpr (W > kRowsMaxW) {
+    throw Error{kErrUnsupported, "nufft: no register row-window spreading for this width"};
+  } else {
+    if (w == W) {
+      NUSHT_HIP(hipFuncSetAttribute((const void*)k_spread_rows<T, W>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lay.bytes));
+      hipLaunchKernelGGL((k_spread_rows<T, W>), grid, dim3(256), lay.bytes, st, a, lay);
+    } else {
+      launch_spread_rows<T, W + 1>(w, grid, st, a, lay);
+    }
+  }
+}
+
 NufftParams NufftParams::choose(int lmax, int spin, double eps, double sigma_hint) {
   NUSHT_REQUIRE(lmax >= 1, kErrArg, "nufft: lmax must be >= 1");
   NUSHT_REQUIRE(eps > 0 && eps < 1, kErrArg, "nufft: epsilon must be in (0, 1)");
@@ -911,7 +1142,8 @@ static void fit_kernel_poly(int w, double beta, double target, bool single, int&
   err = best_err;
 }
 
-NufftPlan::NufftPlan(const NufftParams& p, long npts_max, hipStream_t st) : p_(p), npts_max_(npts_max) {
+NufftPlan::NufftPlan(const NufftParams& p, long npts_max, hipStream_t st, int tile_rows)
+    : p_(p), npts_max_(npts_max) {
   // deconvolution table: 1 / int_{-w/2}^{w/2} psi(2t/w) cos(2 pi k t / nf) dt
   std::vector<double> xq, wq;
   gauss_legendre(4 * p.w + 60, xq, wq);
@@ -927,10 +1159,12 @@ NufftPlan::NufftPlan(const NufftParams& p, long npts_max, hipStream_t st) : p_(p
     dec[k] = 1.0 / (0.5 * p.w * s);
   }
   upload(deconv_, dec);
+  bool fitted = false;
   {
     std::vector<double> c;
     const double target = std::max(p.eps / 100.0, p.single ? 3e-7 : 1e-15);
     fit_kernel_poly(p.w, p.beta, target, p.single, p_.poly_deg, p_.poly_err, c);
+    fitted = !c.empty();
     if (c.empty()) c.push_back(0.0);
     if (p.single) {
       std::vector<float> cf(c.begin(), c.end());
@@ -946,6 +1180,25 @@ NufftPlan::NufftPlan(const NufftParams& p, long npts_max, hipStream_t st) : p_(p
   const size_t cb = p.single ? 8 : 16;
   T_.alloc(cb * (size_t)p.nrow * p.nm);
   grid_.alloc(cb * (size_t)p.nrow * p.nf);
+  // tiles: tile_r_ rows x 16 (17 - w) columns for the register row-window spreading (8 <= w <= 16; the
+  // interpolation takes the same tiles), 32 x 32 otherwise.  NUSHT_SPREAD_ROWS=0 keeps the LDS-atomic
+  // spreading and its tiles; NUSHT_SPREAD_TR sets the tile height (rows of first cells).
+  rows_spread_ = fitted && p.w >= kRowsMinW && p.w <= kRowsMaxW && !(getenv("NUSHT_SPREAD_ROWS") &&
+                                                                     getenv("NUSHT_SPREAD_ROWS")[0] == '0');
+  if (rows_spread_) {
+    int tr = tile_rows > 0 ? tile_rows : 16;
+    if (const char* e = getenv("NUSHT_SPREAD_TR"); tile_rows <= 0 && e && e[0]) tr = atoi(e);
+    tr = std::max(1, std::min(32, tr));
+    tile_r_ = tr;
+    tile_c_ = kRowsGroups * (17 - p.w);
+    const size_t rec = sizeof(double2) + (p.single ? sizeof(float2) : sizeof(double2)) + 1;
+    const RowsLayout base = p.single ? rows_layout<float>(p.w, tr, p_.poly_deg, 0)
+                                     : rows_layout<double>(p.w, tr, p_.poly_deg, 0);
+    const size_t budget = 78 * 1024;  // two workgroups per CU
+    long cap = base.bytes + 64 * rec <= budget ? (long)((budget - base.bytes) / rec) : 64;
+    cap = std::max(64L, std::min(2048L, cap - cap % 64));
+    rows_cap_ = (int)cap;
+  }
   ntile_r_ = ceil_div(p.nrow, tile_r_);
   ntile_c_ = ceil_div(p.nf, tile_c_);
   const int ntiles = ntile_r_ * ntile_c_;
@@ -1072,8 +1325,13 @@ void NufftPlan::type1_impl(const double* ire, const double* iim, const double* g
     SpreadArgs<T> sa{p.nf, p.w, p.row_lo, p.nrow, p.spin, tile_r_, tile_c_, ntile_c_, (T)p.beta, kpoly_.as<T>(),
                      p.poly_deg, pts_.as<double2>(), idx_.as<int>(), tile_start_.as<int>(), ire, iim, gamma,
                      grid_.as<T2>()};
-    const size_t lds = sizeof(double2) * (size_t)(tile_r_ + p.w) * (tile_c_ + p.w);  // FP64 accumulation tile
-    launch_spread<T, 2>(p.w, dim3(ntile_r_ * ntile_c_), lds, st, sa);
+    if (rows_spread_) {
+      const RowsLayout lay = rows_layout<T>(p.w, tile_r_, p.poly_deg, rows_cap_);
+      launch_spread_rows<T, kRowsMinW>(p.w, dim3(ntile_r_ * ntile_c_), st, sa, lay);
+    } else {
+      const size_t lds = sizeof(double2) * (size_t)(tile_r_ + p.w) * (tile_c_ + p.w);  // FP64 accumulation tile
+      launch_spread<T, 2>(p.w, dim3(ntile_r_ * ntile_c_), lds, st, sa);
+    }
     NUSHT_LAUNCH_CHECK();
   }
   PhiArgs<T> pa{p.lmax, p.nf, p.nm, p.nrow, fnf, T_.as<T2>(), grid_.as<T2>()};
@@ -1098,6 +1356,115 @@ void NufftPlan::type1(const double* ire, const double* iim, const double* gamma,
                       hipStream_t st) {
   if (p_.single) type1_impl<float>(ire, iim, gamma, G, sm, st);
   else type1_impl<double>(ire, iim, gamma, G, sm, st);
+}
+
+// ------------------------------------------------------------------ plan-time autotuning
+namespace {
+
+// Uniform points on the sphere from a counter-based hash (splitmix64 finaliser).
+__device__ __forceinline__ double hash_unit(unsigned long long x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  x ^= x >> 31;
+  return (double)(x >> 11) * 0x1p-53;
+}
+
+__global__ void k_random_points(long n, double* __restrict__ theta, double* __restrict__ phi) {
+  for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    theta[i] = acos(1.0 - 2.0 * hash_unit(2 * (unsigned long long)i));
+    phi[i] = 2.0 * M_PI * hash_unit(2 * (unsigned long long)i + 1);
+  }
+}
+
+}  // namespace
+
+std::unique_ptr<NufftPlan> nufft_autotune(int lmax, int spin, double eps, long npts_max, double2* G, long sm,
+                                          hipStream_t st, NufftTune& tune) {
+  int dev = 0;
+  NUSHT_HIP(hipGetDevice(&dev));
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int, double, long>, std::pair<NufftTune, NufftParams>> cache;
+  const auto key = std::make_tuple(dev, lmax, spin, eps, npts_max);
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+      tune = it->second.first;
+      return std::make_unique<NufftPlan>(it->second.second, npts_max, st, tune.tile_rows);
+    }
+  }
+  const long n = std::max<long>(npts_max, 1);
+  DevBuf pts(sizeof(double) * 4 * (size_t)n);  // theta, phi, values re, im
+  double* th = pts.as<double>();
+  double* ph = th + n;
+  double* vr = ph + n;
+  double* vi = vr + n;
+  hipLaunchKernelGGL(k_random_points, dim3(std::min<long>(ceil_div(n, 256), 65536)), dim3(256), 0, st, n, th, ph);
+  NUSHT_LAUNCH_CHECK();
+  NUSHT_HIP(hipMemsetAsync(vr, 0, sizeof(double) * 2 * (size_t)n, st));
+  const int nmrows = spin ? 2 * lmax + 1 : lmax + 1;
+  const NufftParams def = NufftParams::choose(lmax, spin, eps, 0.0);
+  NUSHT_HIP(hipMemsetAsync(G, 0, sizeof(double2) * (size_t)sm * nmrows, st));
+  hipEvent_t e0, e1, e2;
+  NUSHT_HIP(hipEventCreate(&e0));
+  NUSHT_HIP(hipEventCreate(&e1));
+  NUSHT_HIP(hipEventCreate(&e2));
+  const char* env = getenv("NUSHT_VERBOSE");
+  const bool verbose = env && env[0] && env[0] != '0';
+  std::vector<int> tried_nf;
+  float best = 1e30f;
+  NufftTune bt;
+  std::unique_ptr<NufftPlan> best_plan;
+  for (double sg : {def.sigma, 1.25, 1.5, 2.0}) {
+    NufftParams p;
+    try {
+      p = NufftParams::choose(lmax, spin, eps, sg);
+    } catch (const Error&) {
+      continue;
+    }
+    if (std::find(tried_nf.begin(), tried_nf.end(), p.nf) != tried_nf.end()) continue;
+    tried_nf.push_back(p.nf);
+    const bool rows = p.w >= kRowsMinW && p.w <= kRowsMaxW;
+    for (int tr : rows ? std::vector<int>{8, 16, 32} : std::vector<int>{0}) {
+      auto plan = std::make_unique<NufftPlan>(p, n, st, tr);
+      plan->set_pts(th, ph, n, st, false);
+      plan->type2(G, sm, nullptr, vr, spin ? vi : nullptr, st);  // warm-up
+      plan->type1(vr, spin ? vi : nullptr, nullptr, G, sm, st);
+      NUSHT_HIP(hipEventRecord(e0, st));
+      for (int k = 0; k < 2; ++k) plan->type2(G, sm, nullptr, vr, spin ? vi : nullptr, st);
+      NUSHT_HIP(hipEventRecord(e1, st));
+      for (int k = 0; k < 2; ++k) plan->type1(vr, spin ? vi : nullptr, nullptr, G, sm, st);
+      NUSHT_HIP(hipEventRecord(e2, st));
+      NUSHT_HIP(hipEventSynchronize(e2));
+      float t2 = 0, t1 = 0;
+      NUSHT_HIP(hipEventElapsedTime(&t2, e0, e1));
+      NUSHT_HIP(hipEventElapsedTime(&t1, e1, e2));
+      t2 /= 2;
+      t1 /= 2;
+      ++bt.candidates;
+      if (verbose)
+        fprintf(stderr, "[nusht] nufft autotune lmax %d spin %d eps %g: sigma %.4f (nf %d, w %d) tile rows %d: "
+                        "type 2 %.3f ms, type 1 %.3f ms\n", lmax, spin, eps, p.sigma, p.nf, p.w, plan->tile_rows(), t2,
+                t1);
+      if (t2 + t1 < best) {
+        best = t2 + t1;
+        bt.sigma = p.sigma;
+        bt.tile_rows = rows ? tr : 0;
+        bt.type2_ms = t2;
+        bt.type1_ms = t1;
+        best_plan = std::move(plan);
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(e2);
+  NUSHT_REQUIRE(best_plan, kErrUnsupported, "nufft autotune: no candidate fits");
+  tune = bt;
+  std::lock_guard<std::mutex> g(mu);
+  cache[key] = {bt, best_plan->params()};
+  return best_plan;
 }
 
 }  // namespace nusht

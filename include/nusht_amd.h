@@ -161,7 +161,10 @@ typedef struct {
   int ncomp;            /* 1 (spin 0 or GRAD_ONLY) or 2 (E, B) */
   double epsilon;       /* target accuracy; FFT + nuFFT in FP32 when epsilon >= 1e-6 */
   int64_t npts_max;     /* largest point set passed to set_pts */
-  double sigma;         /* fine-grid oversampling, 0 = automatic */
+  double sigma;         /* fine-grid oversampling: > 0 given; 0 automatic (1.25, or 2 below eps 1e-8, as far
+                           as the grid fits the in-LDS transforms); < 0 measured at plan creation (candidates
+                           sigma 1.25, 1.5, 2 x spreading tile heights 8, 16, 32; the fastest type 2 + type 1
+                           on npts_max synthetic points); 0 is measured too when NUSHT_NUFFT_AUTOTUNE=1 */
   int max_maps;         /* maps per nusht_nusht2d2_batch call the plan holds buffers for (1..8; 0 = 8) */
 } nusht_plan_desc;
 
@@ -179,6 +182,11 @@ typedef struct {
                                    which a whole group of 64 ring pairs is below 2^-300 are skipped */
   int legendre_variant;  /* synthesis kernel variant chosen by the plan-time autotuner */
   float legendre_ms;     /* its measured time for one map */
+  int nufft_tile_rows;   /* point tile (fine cells of first footprint cell): rows x cols */
+  int nufft_tile_cols;
+  int nufft_candidates;  /* nuFFT configurations timed at plan creation (0: not tuned) */
+  float nufft_type2_ms;  /* the chosen one's type-2 and type-1 times on the synthetic points */
+  float nufft_type1_ms;
 } nusht_plan_info;
 
 int nusht_plan_create(const nusht_plan_desc* desc, nusht_stream_t s, nusht_plan_t* out);
