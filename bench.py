@@ -4,14 +4,18 @@ Workload (synthetic, seeded): E alm with C_l ~ exp(-(l/3000)^2) / (l(l+1)), B wi
 spectrum; deflection with C_l^dd ~ l^-2 (2 <= l <= lmax) normalised to 2.5 arcmin rms; pointing grid
 GL (lmax+1) x (2 lmax + 1).  All inputs are resident in HBM before the timed region.
 
-Two paths are timed:
+Paths timed:
   plain   : one gclm2lenmap per step, each with its own deflection (spin-1 SHT + pointing,
             point binning, spin-2 nuSHT with the polarisation rotation);
   batched : B maps per step sharing one deflection (pointing once per step, one Legendre
-            recursion for all B maps, then the per-map DFS / nuFFT passes).
-`value` is the batched rate (lensed maps/s); the plain rate, p50/p99 per-call latency, the
-roofline of the dominant kernel (live hipEvent timing inside the library) and an oracle CPU
-baseline are reported beside it.
+            recursion for all B maps, then the per-map DFS / nuFFT passes);
+  batched, own deflection : B maps per step, each with its own deflection, through the batch
+            executor (Legendre launches shared, pointing and binning per map);
+  served  : requests replayed through the continuous-batching server (Poisson arrivals);
+  adjoint : lenmap2gclm one call at a time.
+`value` is the batched rate (lensed maps/s); the other rates, p50/p99 latencies, the roofline of the
+dominant kernel (live hipEvent timing inside the library) and an oracle CPU baseline (one unscaled map of
+the same workload) are reported beside it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--lmax L] [--eps E] [--batch B] [--no-cpu]
        python bench.py --uneven-batch 64      (64 independent transforms of uneven lmax / spin)
@@ -151,6 +155,36 @@ def _host_threads() -> int:
     return min(n, int(env)) if env and env.isdigit() and int(env) > 0 else n
 
 
+def rotated_deflections(dlm, L, n):
+    """n deflections for the same seeded spectrum: dlm with its m-modes' phases turned by m * 0.3 k rad (k = 0..n-1),
+    each a different field of the same statistics."""
+    import torch
+    m = torch.as_tensor(np.concatenate([np.full(L + 1 - mm, mm) for mm in range(L + 1)]), device="cuda")
+    return [dlm * torch.exp(1j * 0.3 * k * m) for k in range(n)]
+
+
+def own_deflection_batch(args, L, eps, gclms, dlms, geominfo):
+    """B lensed maps per step, each with its own deflection, through the batch executor: the Legendre syntheses
+    of the B maps share matrix-core launches (4 maps each) while every map computes its own pointing, binning and
+    interpolation.  The like-for-like counterpart of the plain rate (nothing of the pointing is shared)."""
+    import torch
+    from cunusht_amd.batch import BatchJob, LensingPlan, execute
+    B = args.batch
+    plan = LensingPlan(L, 2, eps, geominfo, max_maps=8)
+    defl = rotated_deflections(dlms[0], L, B)
+    jobs = [BatchJob("gclm2lenmap", plan, defl[k], gclms[k % len(gclms)]) for k in range(B)]
+    for _ in range(args.warmup):
+        execute(jobs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        execute(jobs)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    return {"value": args.steps * B / t, "unit": "lensed maps/s", "maps_per_step": B,
+            "ms_per_step": t / args.steps * 1e3, "deflections_per_step": B}
+
+
 def serve_headline(args, L, eps, gclms, dlms, rng):
     """Replay spin-2 gclm2lenmap requests through the continuous-batching server: Poisson arrivals at
     args.serve_rate, four deflections each shared by a quarter of the requests (same-deflection requests
@@ -161,10 +195,8 @@ def serve_headline(args, L, eps, gclms, dlms, rng):
     geo = lambda lmax: ("gl", {"lmax": lmax, "nphi": 2 * lmax + 1})
     ndefl = 4
     defl = [dlms[i % len(dlms)] for i in range(min(ndefl, len(dlms)))]
-    while len(defl) < ndefl:  # further deflections: the first one with its phases rotated by m * 0.3 rad
-        k = len(defl)
-        m = torch.as_tensor(np.concatenate([np.full(L + 1 - mm, mm) for mm in range(L + 1)]), device="cuda")
-        defl.append(dlms[0] * torch.exp(1j * 0.3 * k * m))
+    if len(defl) < ndefl:  # further deflections: the first one with its phases rotated by m * 0.3 k rad
+        defl += rotated_deflections(dlms[0], L, ndefl)[len(defl):]
     server = LensingServer(hbm_budget_bytes=160 << 30, plans_per_key=2, max_batch=8, geometry=geo)
     key = (L, 2, float(eps), 2)
 
@@ -186,8 +218,9 @@ def serve_headline(args, L, eps, gclms, dlms, rng):
 def uneven_batch(njobs: int, steps: int, warmup: int):
     """Independent transforms of uneven size: lmax ~ U{511, 1023, 1535, 2047}, spin ~ U{0, 2}, eps 1e-7, each
     with its own seeded gclm (C_l ~ (l+1)^-2) and deflection (2.5 arcmin rms, C_l ~ l^-2).  Jobs of one
-    (lmax, spin) share two resident plans.  Timed: the jobs as sequential single calls on one stream, the same
-    jobs through nusht_batch_execute, and the batch captured once as a hipGraph and replayed."""
+    (lmax, spin) share two resident plans that hold 4 maps, so the batch executor runs their Legendre syntheses
+    together on the matrix cores.  Timed: the jobs as sequential single calls on one stream, the same jobs
+    through nusht_batch_execute, and the batch captured once as a hipGraph and replayed."""
     import torch
     from cunusht_amd.batch import BatchJob, CapturedBatch, LensingPlan, execute
     from cunusht_amd.utils import synalm
@@ -197,7 +230,7 @@ def uneven_batch(njobs: int, steps: int, warmup: int):
     plans, jobs = {}, []
     for i, (L, s) in enumerate(zip(lmaxs, spins)):
         L, s = int(L), int(s)
-        pl = plans.setdefault((L, s), [LensingPlan(L, s, 1e-7), LensingPlan(L, s, 1e-7)])
+        pl = plans.setdefault((L, s), [LensingPlan(L, s, 1e-7, max_maps=4), LensingPlan(L, s, 1e-7, max_maps=4)])
         ll = np.arange(L + 1, dtype=np.float64)
         cl = 1.0 / (ll + 1.0) ** 2
         nc = 1 if s == 0 else 2
@@ -549,6 +582,9 @@ def main():
         a = w_i / (interp[0] / interp[1] / 1e3) / 1e9
         nufft_roof = {"achieved_GBs": a, "frac": a / PEAK_HBM_GBS, "avg_ms": interp[0] / interp[1]}
 
+    own = own_deflection_batch(args, L, eps, gclms, dlms, geominfo)
+    print(f"[bench] batched, own deflection per map: {own['value']:.2f} maps/s", file=sys.stderr, flush=True)
+
     served = None
     if args.serve_requests > 0:
         served = serve_headline(args, L, eps, gclms, dlms, rng)
@@ -597,10 +633,12 @@ def main():
                                            "pointing_gl": rinfo["legendre_pair_steps"],
                                            "full_gl": nlm * math.ceil((L + 1) / 2)}},
         "plain": {"value": rate_plain, "p50_ms": float(np.percentile(lat, 50)),
-                  "p99_ms": float(np.percentile(lat, 99))},
+                  "p99_ms": float(np.percentile(lat, 99)), "latencies_ms": [round(float(x), 3) for x in lat]},
         "served": served,
         "adjoint": adjoint,
         "batched_over_plain": rate_batch / rate_plain,
+        "batched_own_deflection": own,
+        "batched_own_deflection_over_plain": own["value"] / rate_plain,
         "served_over_plain": (served["maps_per_s"] / rate_plain) if served else None,
         "accuracy": accuracy,
         "roofline": roof,

@@ -3,9 +3,11 @@
 
 #include <algorithm>
 #include <exception>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <new>
+#include <tuple>
 #include <unordered_map>
 #include <vector>
 
@@ -595,6 +597,26 @@ int find_root(std::vector<int>& par, int a) {
   return a;
 }
 
+// Forward jobs `idx` of one (lmax, spin, ncomp), in order on `st`: their Legendre syntheses share launches (the
+// matrix-core kernel takes up to four maps, whatever their deflections) into the G buffer of the first job's plan,
+// max_maps maps at a time; each job then computes its own pointing and interpolates from its slice of that G.
+void run_fused_forward(const nusht_job* jobs, const std::vector<int>& idx, hipStream_t st) {
+  const nusht_plan_t lead = jobs[idx[0]].plan;
+  const long gmap = (long)lead->nmrows * lead->np.ntheta;
+  for (size_t c0 = 0; c0 < idx.size(); c0 += lead->max_maps) {
+    const int nb = (int)std::min<size_t>(lead->max_maps, idx.size() - c0);
+    std::vector<const double2*> alm(nb);
+    for (int b = 0; b < nb; ++b) alm[b] = reinterpret_cast<const double2*>(jobs[idx[c0 + b]].alm_in);
+    legendre_synth_maps(lead->tab, nb, lead->ncomp, alm.data(), lead->coef.as<double2>(), lead->G.as<double2>(),
+                        lead->np.ntheta, 1, gmap, st);
+    for (int b = 0; b < nb; ++b) {
+      const nusht_job& j = jobs[idx[c0 + b]];
+      const double* ga = lensing_points(j.plan, j.ring, j.dlm_scaled, j.polrot, st);
+      j.plan->nu->type2(lead->G.as<double2>() + b * gmap, lead->np.ntheta, ga, j.map_out0, j.map_out1, st);
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -607,6 +629,23 @@ int nusht_batch_execute(const nusht_job* jobs, int njobs, nusht_stream_t s) {
     // jobs that share a plan or a ring plan form one ordered group (union-find over the handles)
     std::vector<int> par(njobs);
     for (int i = 0; i < njobs; ++i) par[i] = i;
+    // forward jobs of one (lmax, spin, ncomp), three or more, whose first plan holds G for four or more maps:
+    // one fused, ordered group (shared Legendre launches)
+    std::map<std::tuple<int, int, int>, std::vector<int>> fwd;
+    for (int i = 0; i < njobs; ++i)
+      if (jobs[i].kind == NUSHT_JOB_GCLM2LENMAP)
+        fwd[std::make_tuple(jobs[i].plan->lmax, jobs[i].plan->spin, jobs[i].plan->ncomp)].push_back(i);
+    std::vector<int> fused_head(njobs, -1);  // fused group: index of its first job; -1: runs alone
+    std::vector<std::vector<int>> fused;
+    for (auto& kv : fwd) {
+      const std::vector<int>& v = kv.second;
+      if (v.size() < 3 || jobs[v[0]].plan->max_maps < kLegendreMfmaMaps || !jobs[v[0]].plan->tab.mfma) continue;
+      for (int i : v) {
+        fused_head[i] = v[0];
+        par[find_root(par, i)] = find_root(par, v[0]);
+      }
+      fused.push_back(v);
+    }
     std::unordered_map<const void*, int> owner;
     for (int i = 0; i < njobs; ++i)
       for (const void* h : {(const void*)jobs[i].plan, (const void*)jobs[i].ring}) {
@@ -650,7 +689,15 @@ int nusht_batch_execute(const nusht_job* jobs, int njobs, nusht_stream_t s) {
     }
     NUSHT_HIP(hipEventRecord(bs.fork, S(s)));
     for (int k = 0; k < K; ++k) NUSHT_HIP(hipStreamWaitEvent(bs.s[k], bs.fork, 0));
-    for (int i = 0; i < njobs; ++i) run_job(jobs[i], bs.s[stream_of[group[i]]]);
+    for (int i = 0; i < njobs; ++i) {
+      hipStream_t si = bs.s[stream_of[group[i]]];
+      if (fused_head[i] < 0) {
+        run_job(jobs[i], si);
+      } else if (fused_head[i] == i) {
+        for (const std::vector<int>& v : fused)
+          if (v[0] == i) run_fused_forward(jobs, v, si);
+      }
+    }
     for (int k = 0; k < K; ++k) {
       NUSHT_HIP(hipEventRecord(bs.join[k], bs.s[k]));
       NUSHT_HIP(hipStreamWaitEvent(S(s), bs.join[k], 0));

@@ -69,6 +69,15 @@ void legendre_synth(const LegendreTables& t, int nmaps, int ncomp, const double2
                     const double* fl, double2* coef_ws, double2* G, long g_stride_m, long g_stride_r,
                     long g_stride_map, hipStream_t st);
 
+// Per-map alm of a matrix-core launch (up to kLegendreMfmaMaps maps at their own addresses).
+struct AlmPtrs4 {
+  const double2* p[4];
+};
+// The same synthesis for nmaps maps whose alm lie at separate addresses (one pointer per map, host array);
+// G as in legendre_synth.
+void legendre_synth_maps(const LegendreTables& t, int nmaps, int ncomp, const double2* const* alm, double2* coef_ws,
+                         double2* G, long g_stride_m, long g_stride_r, long g_stride_map, hipStream_t st);
+
 // Number of synthesis kernel variants.
 int legendre_variant_count();
 

@@ -448,7 +448,7 @@ __global__ __launch_bounds__(64 * WPB) void k_synth(SynthArgs a) {
 // One thread per (block q of 4 l, chain, k, map b): the four columns of map b at l = l0 + 4q + k, one 32-byte
 // store (consecutive threads write consecutive 32 bytes).  Block row y serves m = y and m = lmax - y, whose
 // tables together have about lmax + 2 rows of l, so no block row is idle.
-__global__ void k_prep4(int lmax, int s, int nmaps, int ncomp, const double2* __restrict__ alm, long stride_map,
+__global__ void k_prep4(int lmax, int s, int nmaps, int ncomp, AlmPtrs4 alm,
                         const double* __restrict__ fl, const long* __restrict__ mofs, const long* __restrict__ mblk,
                         const double* __restrict__ cnorm, double* __restrict__ coef) {
   const int nc = s ? 2 : 1;
@@ -474,7 +474,7 @@ __global__ void k_prep4(int lmax, int s, int nmaps, int ncomp, const double2* __
       if (l <= lmax && b < nmaps) {
         double cl = cnorm[mofs[m] + jj];
         if (fl) cl *= fl[l];
-        const double2* ab = alm + b * stride_map;
+        const double2* ab = b == 0 ? alm.p[0] : b == 1 ? alm.p[1] : b == 2 ? alm.p[2] : alm.p[3];
         const double2 ev = ab[mbase + l];
         const double sl = ((l + m) & 1) ? -1.0 : 1.0;
         if (s == 0) {
@@ -1126,9 +1126,15 @@ static void legendre_synth_chunk(const LegendreTables& t, int nmaps, int ncomp, 
                                  long alm_stride_map, const double* fl, double2* coef_ws, double2* G, long sm, long sr,
                                  long smap, hipStream_t st);
 
-static void legendre_synth_mfma(const LegendreTables& t, int nmaps, int ncomp, const double2* alm,
-                                long alm_stride_map, const double* fl, double2* coef_ws, double2* G, long sm, long sr,
-                                long smap, hipStream_t st);
+static void legendre_synth_mfma(const LegendreTables& t, int nmaps, int ncomp, const AlmPtrs4& alm,
+                                const double* fl, double2* coef_ws, double2* G, long sm, long sr, long smap,
+                                hipStream_t st);
+
+static AlmPtrs4 strided_maps(const double2* alm, long stride, int nmaps) {
+  AlmPtrs4 a{};
+  for (int b = 0; b < 4; ++b) a.p[b] = alm + (b < nmaps ? b : 0) * stride;
+  return a;
+}
 
 void legendre_synth(const LegendreTables& t, int nmaps, int ncomp, const double2* alm, long alm_stride_map,
                     const double* fl, double2* coef_ws, double2* G, long sm, long sr, long smap, hipStream_t st) {
@@ -1139,8 +1145,8 @@ void legendre_synth(const LegendreTables& t, int nmaps, int ncomp, const double2
     const int left = nmaps - b;
     if (t.mfma && left >= 3) {
       const int nb = std::min(kLegendreMfmaMaps, left);
-      legendre_synth_mfma(t, nb, ncomp, alm + b * alm_stride_map, alm_stride_map, fl, coef_ws, G + b * smap, sm, sr,
-                          smap, st);
+      legendre_synth_mfma(t, nb, ncomp, strided_maps(alm + b * alm_stride_map, alm_stride_map, nb), fl, coef_ws,
+                          G + b * smap, sm, sr, smap, st);
       b += nb;
     } else {
       const int nb = std::min(ch, left);
@@ -1151,9 +1157,28 @@ void legendre_synth(const LegendreTables& t, int nmaps, int ncomp, const double2
   }
 }
 
-static void legendre_synth_mfma(const LegendreTables& t, int nmaps, int ncomp, const double2* alm,
-                                long alm_stride_map, const double* fl, double2* coef_ws, double2* G, long sm, long sr,
-                                long smap, hipStream_t st) {
+void legendre_synth_maps(const LegendreTables& t, int nmaps, int ncomp, const double2* const* alm, double2* coef_ws,
+                         double2* G, long sm, long sr, long smap, hipStream_t st) {
+  NUSHT_REQUIRE(nmaps >= 1 && nmaps <= 8, kErrArg, "legendre_synth_maps: nmaps must be in [1, 8]");
+  int b = 0;
+  while (b < nmaps) {
+    const int left = nmaps - b;
+    if (t.mfma && left >= 3) {
+      const int nb = std::min(kLegendreMfmaMaps, left);
+      AlmPtrs4 ap{};
+      for (int k = 0; k < 4; ++k) ap.p[k] = alm[b + (k < nb ? k : 0)];
+      legendre_synth_mfma(t, nb, ncomp, ap, nullptr, coef_ws, G + b * smap, sm, sr, smap, st);
+      b += nb;
+    } else {
+      legendre_synth(t, 1, ncomp, alm[b], 0, nullptr, coef_ws, G + b * smap, sm, sr, smap, st);
+      b += 1;
+    }
+  }
+}
+
+static void legendre_synth_mfma(const LegendreTables& t, int nmaps, int ncomp, const AlmPtrs4& alm,
+                                const double* fl, double2* coef_ws, double2* G, long sm, long sr, long smap,
+                                hipStream_t st) {
   const int L = t.lmax;
   double* coef4 = reinterpret_cast<double*>(coef_ws);
   {
@@ -1161,7 +1186,7 @@ static void legendre_synth_mfma(const LegendreTables& t, int nmaps, int ncomp, c
     const int nc = t.spin ? 2 : 1;
     // block row y covers m = y and m = L - y; its x extent is sized for m = 0 (the longest table)
     dim3 grid(ceil_div((L + 4) / 4 * nc * 16, 256), L / 2 + 1);
-    hipLaunchKernelGGL(k_prep4, grid, dim3(256), 0, st, L, t.spin, nmaps, ncomp, alm, alm_stride_map, fl,
+    hipLaunchKernelGGL(k_prep4, grid, dim3(256), 0, st, L, t.spin, nmaps, ncomp, alm, fl,
                        t.mofs.as<long>(), t.mblk.as<long>(), t.cnorm.as<double>(), coef4);
     NUSHT_LAUNCH_CHECK();
   }

@@ -47,7 +47,7 @@ struct NufftParams {
 // Precomputed tables and per-plan buffers.
 class NufftPlan {
  public:
-  // tile_rows: rows of the register row-window spreading tiles (0: 16, or NUSHT_SPREAD_TR)
+  // tile_rows: rows of the register row-window spreading tiles (0: 8, or NUSHT_SPREAD_TR)
   NufftPlan(const NufftParams& p, long npts_max, hipStream_t st, int tile_rows = 0);
   int tile_rows() const { return tile_r_; }
   int tile_cols() const { return tile_c_; }

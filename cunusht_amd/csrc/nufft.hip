@@ -599,10 +599,52 @@ constexpr int kMaxUnrolledW = 16;
 constexpr int kRowsMinW = 8;
 constexpr int kRowsMaxW = 16;
 constexpr int kRowsGroups = 16;  // 16-lane groups per 256-thread workgroup (= slices per tile)
+constexpr int kRowsMinDeg = 3;   // kernel polynomial degrees with a register-resident Horner
+constexpr int kRowsMaxDeg = 24;
+
+template <int D, typename T>
+__device__ __forceinline__ void horner2(const T (&cf)[kRowsMaxDeg + 1], T ut, T up, T& kt, T& kp) {
+  kt = cf[D];
+  kp = cf[D];
+#pragma unroll
+  for (int k = D - 1; k >= 0; --k) {
+    kt = fma(kt, ut, cf[k]);
+    kp = fma(kp, up, cf[k]);
+  }
+}
+
+// kt, kp = the polynomial of this lane's cell at ut, up; deg is uniform
+template <typename T>
+__device__ __forceinline__ void horner_any(const T (&cf)[kRowsMaxDeg + 1], int deg, T ut, T up, T& kt, T& kp) {
+  switch (deg) {
+    case 3: horner2<3>(cf, ut, up, kt, kp); break;
+    case 4: horner2<4>(cf, ut, up, kt, kp); break;
+    case 5: horner2<5>(cf, ut, up, kt, kp); break;
+    case 6: horner2<6>(cf, ut, up, kt, kp); break;
+    case 7: horner2<7>(cf, ut, up, kt, kp); break;
+    case 8: horner2<8>(cf, ut, up, kt, kp); break;
+    case 9: horner2<9>(cf, ut, up, kt, kp); break;
+    case 10: horner2<10>(cf, ut, up, kt, kp); break;
+    case 11: horner2<11>(cf, ut, up, kt, kp); break;
+    case 12: horner2<12>(cf, ut, up, kt, kp); break;
+    case 13: horner2<13>(cf, ut, up, kt, kp); break;
+    case 14: horner2<14>(cf, ut, up, kt, kp); break;
+    case 15: horner2<15>(cf, ut, up, kt, kp); break;
+    case 16: horner2<16>(cf, ut, up, kt, kp); break;
+    case 17: horner2<17>(cf, ut, up, kt, kp); break;
+    case 18: horner2<18>(cf, ut, up, kt, kp); break;
+    case 19: horner2<19>(cf, ut, up, kt, kp); break;
+    case 20: horner2<20>(cf, ut, up, kt, kp); break;
+    case 21: horner2<21>(cf, ut, up, kt, kp); break;
+    case 22: horner2<22>(cf, ut, up, kt, kp); break;
+    case 23: horner2<23>(cf, ut, up, kt, kp); break;
+    default: horner2<24>(cf, ut, up, kt, kp); break;
+  }
+}
 
 struct RowsLayout {
   int tr, tc, aw, cap, deg, nkey;
-  size_t acc_off, val_off, uu_off, off_off, cnt_off, coef_off, bytes;
+  size_t acc_off, val_off, uu_off, off_off, cnt_off, ktb_off, bytes;
 };
 
 template <typename T>
@@ -623,7 +665,7 @@ __host__ __device__ inline RowsLayout rows_layout(int w, int tr, int deg, int ca
   o = (o + 15) & ~(size_t)15;
   L.cnt_off = o; o += sizeof(int) * (size_t)(2 * L.nkey + 1);
   o = (o + 15) & ~(size_t)15;
-  L.coef_off = o; o += sizeof(T) * 16 * (size_t)(deg + 1);
+  L.ktb_off = o; o += sizeof(T) * 16 * kRowsGroups;
   L.bytes = o;
   return L;
 }
@@ -640,17 +682,13 @@ __global__ __launch_bounds__(256) void k_spread_rows(SpreadArgs<T> a, RowsLayout
   unsigned char* offs = smem_raw + lay.off_off;
   int* cnt = reinterpret_cast<int*>(smem_raw + lay.cnt_off);  // [nkey]: counts, then cursors
   int* start = cnt + lay.nkey;                                 // [nkey + 1]
-  T* coef = reinterpret_cast<T*>(smem_raw + lay.coef_off);     // [(deg + 1) x 16], cells >= W zero
+  T* ktbuf = reinterpret_cast<T*>(smem_raw + lay.ktb_off);     // [16 groups][16] row weights of a point
   const int tb = blockIdx.x;
   const int s0 = a.tile_start[tb], s1 = a.tile_start[tb + 1];
   if (s0 == s1) return;
   const int tr = lay.tr, aw = lay.aw, nkey = lay.nkey, deg = lay.deg;
   const int nacc = (tr + W - 1) * aw;
   for (int i = threadIdx.x; i < 2 * nacc; i += blockDim.x) acc[i] = 0.0;
-  for (int i = threadIdx.x; i < 16 * (deg + 1); i += blockDim.x) {
-    const int k = i >> 4, j = i & 15;
-    coef[i] = j < W ? a.kpoly[(long)k * W + j] : T(0);
-  }
   const int tile_r = tb / a.ntc, tile_c = tb % a.ntc;
   const int r0 = tile_r * a.tr, c0 = tile_c * a.tc;
   const double scale = a.nf / (2.0 * M_PI);
@@ -659,6 +697,11 @@ __global__ __launch_bounds__(256) void k_spread_rows(SpreadArgs<T> a, RowsLayout
   const int l16 = lane & 15, gsub = lane >> 4;
   const int slice = wave + 4 * gsub;  // the four groups of a wave own non-overlapping windows
   const int gbase = threadIdx.x & ~15;  // first thread of this lane's group
+  // the polynomial of footprint cell l16 (zero for l16 >= W and above the degree), in registers
+  T cf[kRowsMaxDeg + 1];
+#pragma unroll
+  for (int k = 0; k <= kRowsMaxDeg; ++k) cf[k] = (l16 < W && k <= deg) ? a.kpoly[(long)k * W + l16] : T(0);
+  T* ktb = ktbuf + 16 * (threadIdx.x >> 4);
   // first cells of a binned point, relative to the tile; lr / lc outside the tile never happen
   auto cells = [&](int p, int& lr, int& lc, double& ut, double& up, int& i0, int& j0u) {
     const double2 tp = a.pts[p];
@@ -748,13 +791,16 @@ __global__ __launch_bounds__(256) void k_spread_rows(SpreadArgs<T> a, RowsLayout
               const double2 v = val[q];
               const T2 u = uu[q];
               const int off = offs[q];
-              const T* c = coef + deg * 16 + l16;
-              T kt = *c, kp = kt;
-              for (int k = deg - 1; k >= 0; --k) {
-                c -= 16;
-                kt = fma(kt, u.x, *c);
-                kp = fma(kp, u.y, *c);
-              }
+              T kt, kp;
+              horner_any(cf, deg, u.x, u.y, kt, kp);
+              // the row weights reach the group's lanes through its LDS slot (LDS operations of one wave
+              // complete in order; the fences keep the compiler from moving the reads above the write)
+              ktb[l16] = kt;
+              __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+              __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+              T ktv[W];
+#pragma unroll
+              for (int i = 0; i < W; ++i) ktv[i] = ktb[i];
               const int src = l16 - off;  // footprint cell of this lane's column
               const T kpc_ = __shfl(kp, gbase + (src & 15), 64);
               const double kpc = (src >= 0 && src < W) ? (double)kpc_ : 0.0;
@@ -762,7 +808,7 @@ __global__ __launch_bounds__(256) void k_spread_rows(SpreadArgs<T> a, RowsLayout
               static_for<0, W>([&](auto ic) {
                 constexpr int i = decltype(ic)::value;
                 constexpr int slot = (K + i) % W;
-                const double kti = (double)row_bcast<i>(kt);
+                const double kti = (double)ktv[i];
                 ar[slot] = fma(cr, kti, ar[slot]);
                 ai[slot] = fma(ci, kti, ai[slot]);
               });
@@ -1180,22 +1226,26 @@ NufftPlan::NufftPlan(const NufftParams& p, long npts_max, hipStream_t st, int ti
   const size_t cb = p.single ? 8 : 16;
   T_.alloc(cb * (size_t)p.nrow * p.nm);
   grid_.alloc(cb * (size_t)p.nrow * p.nf);
-  // tiles: tile_r_ rows x 16 (17 - w) columns for the register row-window spreading (8 <= w <= 16; the
+  // tiles: tile_r_ rows (default 8) x 16 (17 - w) columns for the register row-window spreading (8 <= w <= 16; the
   // interpolation takes the same tiles), 32 x 32 otherwise.  NUSHT_SPREAD_ROWS=0 keeps the LDS-atomic
   // spreading and its tiles; NUSHT_SPREAD_TR sets the tile height (rows of first cells).
-  rows_spread_ = fitted && p.w >= kRowsMinW && p.w <= kRowsMaxW && !(getenv("NUSHT_SPREAD_ROWS") &&
-                                                                     getenv("NUSHT_SPREAD_ROWS")[0] == '0');
+  rows_spread_ = fitted && p.w >= kRowsMinW && p.w <= kRowsMaxW && p_.poly_deg >= kRowsMinDeg &&
+                 p_.poly_deg <= kRowsMaxDeg &&
+                 !(getenv("NUSHT_SPREAD_ROWS") && getenv("NUSHT_SPREAD_ROWS")[0] == '0');
   if (rows_spread_) {
-    int tr = tile_rows > 0 ? tile_rows : 16;
+    int tr = tile_rows > 0 ? tile_rows : 8;  // the measured best at lmax 2047 and 4095 (eps 1e-6, 1e-7)
     if (const char* e = getenv("NUSHT_SPREAD_TR"); tile_rows <= 0 && e && e[0]) tr = atoi(e);
     tr = std::max(1, std::min(32, tr));
     tile_r_ = tr;
     tile_c_ = kRowsGroups * (17 - p.w);
+    // LDS: the accumulation tile plus point records, for two workgroups per CU when 512 or more records fit
+    // beside the tile in 78 KiB, else one workgroup with up to 156 KiB
     const size_t rec = sizeof(double2) + (p.single ? sizeof(float2) : sizeof(double2)) + 1;
     const RowsLayout base = p.single ? rows_layout<float>(p.w, tr, p_.poly_deg, 0)
                                      : rows_layout<double>(p.w, tr, p_.poly_deg, 0);
-    const size_t budget = 78 * 1024;  // two workgroups per CU
-    long cap = base.bytes + 64 * rec <= budget ? (long)((budget - base.bytes) / rec) : 64;
+    const size_t budget = base.bytes + 512 * rec <= 78 * 1024 ? 78 * 1024 : 156 * 1024;
+    NUSHT_REQUIRE(base.bytes + 64 * rec <= budget, kErrUnsupported, "nufft: spreading tile too large for LDS");
+    long cap = (long)((budget - base.bytes) / rec);
     cap = std::max(64L, std::min(2048L, cap - cap % 64));
     rows_cap_ = (int)cap;
   }
@@ -1425,9 +1475,9 @@ std::unique_ptr<NufftPlan> nufft_autotune(int lmax, int spin, double eps, long n
     }
     if (std::find(tried_nf.begin(), tried_nf.end(), p.nf) != tried_nf.end()) continue;
     tried_nf.push_back(p.nf);
-    const bool rows = p.w >= kRowsMinW && p.w <= kRowsMaxW;
-    for (int tr : rows ? std::vector<int>{8, 16, 32} : std::vector<int>{0}) {
+    for (int tr : {16, 8, 32}) {
       auto plan = std::make_unique<NufftPlan>(p, n, st, tr);
+      const bool rows = plan->rows_spread();  // otherwise the tile height is fixed: time one plan
       plan->set_pts(th, ph, n, st, false);
       plan->type2(G, sm, nullptr, vr, spin ? vi : nullptr, st);  // warm-up
       plan->type1(vr, spin ? vi : nullptr, nullptr, G, sm, st);
@@ -1444,8 +1494,9 @@ std::unique_ptr<NufftPlan> nufft_autotune(int lmax, int spin, double eps, long n
       t1 /= 2;
       ++bt.candidates;
       if (verbose)
-        fprintf(stderr, "[nusht] nufft autotune lmax %d spin %d eps %g: sigma %.4f (nf %d, w %d) tile rows %d: "
-                        "type 2 %.3f ms, type 1 %.3f ms\n", lmax, spin, eps, p.sigma, p.nf, p.w, plan->tile_rows(), t2,
+        fprintf(stderr, "[nusht] nufft autotune lmax %d spin %d eps %g: sigma %.4f (nf %d, w %d, degree %d) tile %d x %d "
+                        "(%s spreading): type 2 %.3f ms, type 1 %.3f ms\n", lmax, spin, eps, p.sigma, p.nf, p.w,
+                plan->params().poly_deg, plan->tile_rows(), plan->tile_cols(), rows ? "row-window" : "LDS-atomic", t2,
                 t1);
       if (t2 + t1 < best) {
         best = t2 + t1;
@@ -1455,6 +1506,7 @@ std::unique_ptr<NufftPlan> nufft_autotune(int lmax, int spin, double eps, long n
         bt.type1_ms = t1;
         best_plan = std::move(plan);
       }
+      if (!rows) break;
     }
   }
   (void)hipEventDestroy(e0);

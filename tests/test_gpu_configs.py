@@ -12,8 +12,9 @@ sampled colatitudes, then the sum over m), and the dot-product adjointness of ea
   deflection summed directly there, then the closed-form pointing and gamma), within 30 eps in FP32 and 1e-7
   (the headline bar) in FP64; the gclm2lenmap / lenmap2gclm pair adjoint to 1e-5 (FP32) / 1e-9 (FP64).
 * 64 transforms of lmax {511, 1023, 1535, 2047} x spin {0, 2}, eps 1e-7, each with its own gclm and deflection,
-  as one hipGraph-captured batch replay: every job equal to its single call (1e-13), 200 pixels of four jobs
-  against direct summation at the oracle's pointing.
+  as one hipGraph-captured batch replay (the Legendre syntheses of each (lmax, spin) fused onto the matrix
+  cores): every job equal to its single call (1e-13), 200 pixels of four jobs against direct summation at the
+  oracle's pointing.
 * The serving loop on a Poisson mix of lmax 255 .. 4095, spin 0 / 2, eps 1e-5 / 1e-7 / 1e-10, forward and
   adjoint: every result equal to the same request run alone (1e-13 forward / 1e-12 adjoint in FP64, eps with
   the FP32 nuFFT), and one forward request per (lmax, spin) against direct summation at 100 pixels.
@@ -187,7 +188,7 @@ def test_config_uneven_batch_64_jobs_graph():
     plans, jobs, host = {}, [], []
     for i, (L, s) in enumerate(zip(lmaxs, spins)):
         L, s = int(L), int(s)
-        pl = plans.setdefault((L, s), [LensingPlan(L, s, 1e-7), LensingPlan(L, s, 1e-7)])
+        pl = plans.setdefault((L, s), [LensingPlan(L, s, 1e-7, max_maps=4), LensingPlan(L, s, 1e-7, max_maps=4)])
         nc = 1 if s == 0 else 2
         cl = 1.0 / (np.arange(L + 1) + 1.0) ** 2
         g = np.stack([almutil.synalm(cl, L, rng) for _ in range(nc)])

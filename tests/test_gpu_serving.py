@@ -165,6 +165,36 @@ def test_server_results_same_alone_and_in_mix():
         assert _relmax(got.reshape(ref.shape), ref) < tol, (r.kind, r.lmax, r.spin)
 
 
+@pytest.mark.parametrize("spin,eps", [(2, 1e-10), (0, 1e-7), (2, 1e-5)])
+def test_fused_forward_batch_equals_single_calls(spin, eps):
+    """Seven forward jobs of one (lmax, spin), each with its own deflection, on two plans that hold 4 maps:
+    the executor runs their Legendre syntheses together (4 + 3 maps on the matrix cores) and each job's pointing
+    and interpolation from its slice; every result equals the job's single call.  A captured replay too."""
+    from cunusht_amd.batch import BatchJob, CapturedBatch, LensingPlan, execute
+    L = 36
+    rng = np.random.default_rng(970 + spin)
+    plans = [LensingPlan(L, spin, eps, max_maps=4), LensingPlan(L, spin, eps, max_maps=4)]
+    nc = 1 if spin == 0 else 2
+    jobs = [BatchJob("gclm2lenmap", plans[k % 2], _t(_deflection(L, rng)),
+                     _t(almutil.random_alm(L, spin, nc, rng))) for k in range(7)]
+    print(f"case fused forward batch spin={spin} eps={eps}", flush=True)
+    execute(jobs)
+    torch.cuda.synchronize()
+    got = [j.out.cpu().numpy() for j in jobs]
+    tol = 1e-6 if eps >= 1e-6 else 1e-13
+    for k, j in enumerate(jobs):
+        ref = _alone(j.plan, "gclm2lenmap", j.inp, j.dlm_scaled)
+        assert _relmax(got[k], ref) < tol, k
+    cap = CapturedBatch(jobs)
+    for j in jobs:
+        j.out.fill_(np.nan)
+    for _ in range(2):
+        cap.replay()
+        torch.cuda.synchronize()
+        for k, j in enumerate(jobs):
+            assert _relmax(j.out.cpu().numpy(), got[k]) < tol, k
+
+
 def test_server_budget_counts_staging_and_outputs():
     """A slot's bytes include its batch staging and one launch's outputs: with a budget that holds the plans
     of two slots but not their staging, a configuration gets one slot, and every request still completes."""
