@@ -483,9 +483,9 @@ def main():
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for i in range(args.steps):
+    for i in range(args.steps):  # deflections alternate from the last warm-up one: every call computes its pointing
         evs[i][0].record()
-        plain_step(i + 1)
+        plain_step(i + args.warmup)
         evs[i][1].record()
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0

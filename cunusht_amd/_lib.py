@@ -52,7 +52,7 @@ class PlanInfo(ctypes.Structure):
                 ("single_precision", _i), ("sigma", _d), ("beta", _d), ("workspace_bytes", _i64),
                 ("legendre_pair_steps", _i64), ("legendre_variant", _i), ("legendre_ms", ctypes.c_float),
                 ("nufft_tile_rows", _i), ("nufft_tile_cols", _i), ("nufft_candidates", _i),
-                ("nufft_type2_ms", ctypes.c_float), ("nufft_type1_ms", ctypes.c_float)]
+                ("nufft_type2_ms", ctypes.c_float), ("nufft_type1_ms", ctypes.c_float), ("nufft_spread_rows", _i)]
 
 
 class RingInfo(ctypes.Structure):

@@ -305,6 +305,7 @@ int nusht_plan_get_info(nusht_plan_t p, nusht_plan_info* info) {
     info->nufft_candidates = p->tune.candidates;
     info->nufft_type2_ms = p->tune.type2_ms;
     info->nufft_type1_ms = p->tune.type1_ms;
+    info->nufft_spread_rows = p->nu->rows_spread() ? 1 : 0;
   });
 }
 

@@ -187,6 +187,7 @@ typedef struct {
   int nufft_candidates;  /* nuFFT configurations timed at plan creation (0: not tuned) */
   float nufft_type2_ms;  /* the chosen one's type-2 and type-1 times on the synthetic points */
   float nufft_type1_ms;
+  int nufft_spread_rows; /* 1: register row-window spreading (8 <= w <= 16), 0: LDS-atomic spreading */
 } nusht_plan_info;
 
 int nusht_plan_create(const nusht_plan_desc* desc, nusht_stream_t s, nusht_plan_t* out);

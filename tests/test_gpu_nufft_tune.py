@@ -83,8 +83,10 @@ def test_every_tuner_sigma_at_1e7(sigma, spin):
 
 
 def _rows_kernel_used(info):
-    """The plan spreads with the register row-window kernel (its tiles are 16 (17 - w) columns wide)."""
-    return info["nufft_tile_cols"] == 16 * (17 - info["kernel_width"])
+    """The plan spreads with the register row-window kernel (tiles 16 (17 - w) columns wide)."""
+    used = info["nufft_spread_rows"] == 1
+    assert not used or info["nufft_tile_cols"] == 16 * (17 - info["kernel_width"])
+    return used
 
 
 @pytest.mark.parametrize("rows", [1, 4, 8, 16, 32])
