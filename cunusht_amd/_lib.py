@@ -86,6 +86,7 @@ SIGNATURES = {
     "nusht_pointplan_create": [_i, _i, _i64, _vp, _vp, ctypes.POINTER(_vp)],
     "nusht_pointplan_destroy": [_vp],
     "nusht_pointplan_synthesis": [_vp, _vp, _i, _vp, _vp, _vp, _vp, _vp],
+    "nusht_pointplan_synthesis_noalloc": [_i, _i, _i64, _vp, _vp, _i, _vp, _vp, _vp, _vp, _vp],
     "nusht_plan_create": [ctypes.POINTER(PlanDesc), _vp, ctypes.POINTER(_vp)],
     "nusht_plan_destroy": [_vp],
     "nusht_plan_get_info": [_vp, ctypes.POINTER(PlanInfo)],

@@ -151,6 +151,15 @@ int nusht_pointplan_synthesis(nusht_pointplan_t p, const double* alm, int ncomp,
   });
 }
 
+int nusht_pointplan_synthesis_noalloc(int lmax, int spin, int64_t npts, const double* theta, const double* alm,
+                                      int ncomp, const double* phi, const double* gamma, double* out0, double* out1,
+                                      nusht_stream_t s) {
+  return guard([&] {
+    point_synthesis_noalloc(lmax, spin, npts, theta, reinterpret_cast<const double2*>(alm), ncomp, phi, gamma, out0,
+                            out1, S(s));
+  });
+}
+
 // ------------------------------------------------------------------ ring plan
 int nusht_ringplan_create(const nusht_ring_desc* d, nusht_stream_t s, nusht_ringplan_t* out) {
   return guard([&] {

@@ -29,4 +29,13 @@ class PointPlan {
   DevBuf G_, coef_;
 };
 
+// Exact synthesis at points without any device allocation (no plan): the recursion seeds and coefficients are
+// computed on the device per call and the ring coefficients G_m(theta_p) go into a file-scope device array of
+// kNoAllocG complex values, one per GPU, shared by every call on that GPU (calls that overlap on one GPU race).
+// theta, phi, gamma (may be null), alm, out0 / out1 are device arrays; requires (rows = lmax + 1 for spin 0,
+// 2 lmax + 1 otherwise) x npts <= kNoAllocG.
+constexpr long kNoAllocG = 1L << 21;  // 32 MiB of double2
+void point_synthesis_noalloc(int lmax, int spin, long npts, const double* theta, const double2* alm, int ncomp,
+                             const double* phi, const double* gamma, double* out0, double* out1, hipStream_t st);
+
 }  // namespace nusht

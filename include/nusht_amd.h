@@ -149,6 +149,19 @@ int nusht_pointplan_create(int lmax, int spin, int64_t npts, const double* theta
 int nusht_pointplan_destroy(nusht_pointplan_t p);
 int nusht_pointplan_synthesis(nusht_pointplan_t p, const double* alm, int ncomp, const double* phi,
                               const double* gamma, double* out0, double* out1, nusht_stream_t s);
+/* Allocation-free variant of nusht_pointplan_synthesis: no counterpart in the reference.  For callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): no plan, no hipMalloc / hipFree, nothing allocated by any library routine.  theta, phi,
+ * gamma (may be NULL), alm (ncomp x nalm complex, healpy layout), out0 / out1 are device arrays; theta in
+ * [0, pi].  The ring coefficients live in one 32 MiB array per GPU inside the library (2^21 complex
+ * values): (lmax + 1) x npts for spin 0, (2 lmax + 1) x npts for spin > 0 must not exceed 2^21 (e.g. 256
+ * points at lmax 4095 spin 2, 1024 at lmax 1023), or NUSHT_ERR_UNSUPPORTED.  Every call on one GPU uses
+ * that GPU's array: calls that overlap on one GPU (different streams, threads) race and give wrong
+ * results; calls ordered on one stream are safe.  Same results as nusht_pointplan_synthesis (FP64,
+ * different summation order). */
+int nusht_pointplan_synthesis_noalloc(int lmax, int spin, int64_t npts, const double* theta, const double* alm,
+                                      int ncomp, const double* phi, const double* gamma, double* out0, double* out1,
+                                      nusht_stream_t s);
 
 /* ---------------------------------------------------------------- nuSHT plan
  * Replaces GPU_cufinufft_transformer.__init__ / plan (cunusht/deflection/GPU_nufft_transformer.py:108-173,
