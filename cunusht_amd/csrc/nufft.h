@@ -84,6 +84,7 @@ class NufftPlan {
   int tile_r_ = 32, tile_c_ = 32;  // spreading tile (fine cells)
   bool rows_spread_ = false;       // register row-window spreading (8 <= w <= 16)
   int rows_cap_ = 0;               // its LDS point records per pass
+  int rows_seg_ = 1;               // tiles one workgroup walks down a column strip
   int ntile_r_ = 0, ntile_c_ = 0;
   // tables
   DevBuf deconv_;                  // double[lmax+1] : 1 / psi_hat(k)

@@ -599,11 +599,12 @@ constexpr int kMaxUnrolledW = 16;
 constexpr int kRowsMinW = 8;
 constexpr int kRowsMaxW = 16;
 constexpr int kRowsGroups = 16;  // 16-lane groups per 256-thread workgroup (= slices per tile)
+constexpr int kRowsPts = 3;      // points per thread in one pass of the row-window spreading (records <= 768)
 constexpr int kRowsMinDeg = 3;   // kernel polynomial degrees with a register-resident Horner
 constexpr int kRowsMaxDeg = 24;
 
-template <int D, typename T>
-__device__ __forceinline__ void horner2(const T (&cf)[kRowsMaxDeg + 1], T ut, T up, T& kt, T& kp) {
+template <int D, int DM, typename T>
+__device__ __forceinline__ void horner2(const T (&cf)[DM + 1], T ut, T up, T& kt, T& kp) {
   kt = cf[D];
   kp = cf[D];
 #pragma unroll
@@ -613,37 +614,38 @@ __device__ __forceinline__ void horner2(const T (&cf)[kRowsMaxDeg + 1], T ut, T 
   }
 }
 
-// kt, kp = the polynomial of this lane's cell at ut, up; deg is uniform
-template <typename T>
-__device__ __forceinline__ void horner_any(const T (&cf)[kRowsMaxDeg + 1], int deg, T ut, T up, T& kt, T& kp) {
+// kt, kp = the polynomial of this lane's cell at ut, up; deg (<= DM) is uniform
+template <int DM, typename T>
+__device__ __forceinline__ void horner_any(const T (&cf)[DM + 1], int deg, T ut, T up, T& kt, T& kp) {
   switch (deg) {
-    case 3: horner2<3>(cf, ut, up, kt, kp); break;
-    case 4: horner2<4>(cf, ut, up, kt, kp); break;
-    case 5: horner2<5>(cf, ut, up, kt, kp); break;
-    case 6: horner2<6>(cf, ut, up, kt, kp); break;
-    case 7: horner2<7>(cf, ut, up, kt, kp); break;
-    case 8: horner2<8>(cf, ut, up, kt, kp); break;
-    case 9: horner2<9>(cf, ut, up, kt, kp); break;
-    case 10: horner2<10>(cf, ut, up, kt, kp); break;
-    case 11: horner2<11>(cf, ut, up, kt, kp); break;
-    case 12: horner2<12>(cf, ut, up, kt, kp); break;
-    case 13: horner2<13>(cf, ut, up, kt, kp); break;
-    case 14: horner2<14>(cf, ut, up, kt, kp); break;
-    case 15: horner2<15>(cf, ut, up, kt, kp); break;
-    case 16: horner2<16>(cf, ut, up, kt, kp); break;
-    case 17: horner2<17>(cf, ut, up, kt, kp); break;
-    case 18: horner2<18>(cf, ut, up, kt, kp); break;
-    case 19: horner2<19>(cf, ut, up, kt, kp); break;
-    case 20: horner2<20>(cf, ut, up, kt, kp); break;
-    case 21: horner2<21>(cf, ut, up, kt, kp); break;
-    case 22: horner2<22>(cf, ut, up, kt, kp); break;
-    case 23: horner2<23>(cf, ut, up, kt, kp); break;
-    default: horner2<24>(cf, ut, up, kt, kp); break;
+    case 3: if constexpr (3 <= DM) horner2<3, DM>(cf, ut, up, kt, kp); break;
+    case 4: if constexpr (4 <= DM) horner2<4, DM>(cf, ut, up, kt, kp); break;
+    case 5: if constexpr (5 <= DM) horner2<5, DM>(cf, ut, up, kt, kp); break;
+    case 6: if constexpr (6 <= DM) horner2<6, DM>(cf, ut, up, kt, kp); break;
+    case 7: if constexpr (7 <= DM) horner2<7, DM>(cf, ut, up, kt, kp); break;
+    case 8: if constexpr (8 <= DM) horner2<8, DM>(cf, ut, up, kt, kp); break;
+    case 9: if constexpr (9 <= DM) horner2<9, DM>(cf, ut, up, kt, kp); break;
+    case 10: if constexpr (10 <= DM) horner2<10, DM>(cf, ut, up, kt, kp); break;
+    case 11: if constexpr (11 <= DM) horner2<11, DM>(cf, ut, up, kt, kp); break;
+    case 12: if constexpr (12 <= DM) horner2<12, DM>(cf, ut, up, kt, kp); break;
+    case 13: if constexpr (13 <= DM) horner2<13, DM>(cf, ut, up, kt, kp); break;
+    case 14: if constexpr (14 <= DM) horner2<14, DM>(cf, ut, up, kt, kp); break;
+    case 15: if constexpr (15 <= DM) horner2<15, DM>(cf, ut, up, kt, kp); break;
+    case 16: if constexpr (16 <= DM) horner2<16, DM>(cf, ut, up, kt, kp); break;
+    case 17: if constexpr (17 <= DM) horner2<17, DM>(cf, ut, up, kt, kp); break;
+    case 18: if constexpr (18 <= DM) horner2<18, DM>(cf, ut, up, kt, kp); break;
+    case 19: if constexpr (19 <= DM) horner2<19, DM>(cf, ut, up, kt, kp); break;
+    case 20: if constexpr (20 <= DM) horner2<20, DM>(cf, ut, up, kt, kp); break;
+    case 21: if constexpr (21 <= DM) horner2<21, DM>(cf, ut, up, kt, kp); break;
+    case 22: if constexpr (22 <= DM) horner2<22, DM>(cf, ut, up, kt, kp); break;
+    case 23: if constexpr (23 <= DM) horner2<23, DM>(cf, ut, up, kt, kp); break;
+    default: horner2<DM, DM>(cf, ut, up, kt, kp); break;
   }
 }
 
 struct RowsLayout {
   int tr, tc, aw, cap, deg, nkey;
+  int seg, ntr;  // tiles one workgroup walks down its column strip; tile rows of the grid
   size_t acc_off, val_off, uu_off, off_off, cnt_off, ktb_off, bytes;
 };
 
@@ -657,6 +659,8 @@ __host__ __device__ inline RowsLayout rows_layout(int w, int tr, int deg, int ca
   L.cap = cap;
   L.deg = deg;
   L.nkey = kRowsGroups * tr;
+  L.seg = 1;
+  L.ntr = 0;
   size_t o = 0;
   L.acc_off = o; o += sizeof(double2) * (size_t)(tr + w - 1) * L.aw;
   L.val_off = o; o += sizeof(double2) * (size_t)cap;
@@ -670,40 +674,47 @@ __host__ __device__ inline RowsLayout rows_layout(int w, int tr, int deg, int ca
   return L;
 }
 
-template <typename T, int W>
+template <typename T, int W, int DM>
 __global__ __launch_bounds__(256) void k_spread_rows(SpreadArgs<T> a, RowsLayout lay) {
   static_assert(W >= kRowsMinW && W <= kRowsMaxW, "register row windows for 8 <= W <= 16");
   constexpr int SW = 17 - W;
   using T2 = c2_t<T>;
   extern __shared__ unsigned char smem_raw[];
   double* acc = reinterpret_cast<double*>(smem_raw + lay.acc_off);  // interleaved re, im; (tr + W - 1) x aw
-  double2* val = reinterpret_cast<double2*>(smem_raw + lay.val_off);
-  T2* uu = reinterpret_cast<T2*>(smem_raw + lay.uu_off);
-  unsigned char* offs = smem_raw + lay.off_off;
+  double2* val = reinterpret_cast<double2*>(smem_raw + lay.val_off);  // rotated value per point record
+  T2* uu = reinterpret_cast<T2*>(smem_raw + lay.uu_off);               // its row and column polynomial arguments
+  unsigned char* offs = smem_raw + lay.off_off;                        // its first column within the slice
   int* cnt = reinterpret_cast<int*>(smem_raw + lay.cnt_off);  // [nkey]: counts, then cursors
   int* start = cnt + lay.nkey;                                 // [nkey + 1]
   T* ktbuf = reinterpret_cast<T*>(smem_raw + lay.ktb_off);     // [16 groups][16] row weights of a point
-  const int tb = blockIdx.x;
-  const int s0 = a.tile_start[tb], s1 = a.tile_start[tb + 1];
-  if (s0 == s1) return;
+  // one workgroup walks tiles t0 .. t1 - 1 of column strip tile_c; rows a tile has completed leave the ring
+  // buffer acc (R = tr + W - 1 rows, slot = row - t0 tr mod R) while the W - 1 rows below carry over
+  const int tile_c = blockIdx.x % a.ntc;
+  const int t0 = (blockIdx.x / a.ntc) * lay.seg, t1 = min(t0 + lay.seg, lay.ntr);
+  if (a.tile_start[t0 * a.ntc + tile_c] == a.tile_start[(t1 - 1) * a.ntc + tile_c + 1]) return;  // no points
   const int tr = lay.tr, aw = lay.aw, nkey = lay.nkey, deg = lay.deg;
-  const int nacc = (tr + W - 1) * aw;
-  for (int i = threadIdx.x; i < 2 * nacc; i += blockDim.x) acc[i] = 0.0;
-  const int tile_r = tb / a.ntc, tile_c = tb % a.ntc;
-  const int r0 = tile_r * a.tr, c0 = tile_c * a.tc;
+  const int R = tr + W - 1;
+  for (int i = threadIdx.x; i < 2 * R * aw; i += blockDim.x) acc[i] = 0.0;
+  __syncthreads();
+  const int c0 = tile_c * a.tc;
+  // cells no other workgroup touches: columns W - 1 .. tc - 1 of the strip (when there are two strips or
+  // more), rows from W - 1 below the segment's first row (all rows of the first segment) up to the next
+  // segment's first row
+  const int excl_c1 = a.ntc >= 2 ? min(a.tc, a.nf - c0) : 0;
+  const int excl_r0 = t0 == 0 ? 0 : t0 * tr + W - 1, excl_r1 = t1 * tr;
   const double scale = a.nf / (2.0 * M_PI);
   const double hw = 0.5 * W;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int l16 = lane & 15, gsub = lane >> 4;
   const int slice = wave + 4 * gsub;  // the four groups of a wave own non-overlapping windows
   const int gbase = threadIdx.x & ~15;  // first thread of this lane's group
-  // the polynomial of footprint cell l16 (zero for l16 >= W and above the degree), in registers
-  T cf[kRowsMaxDeg + 1];
-#pragma unroll
-  for (int k = 0; k <= kRowsMaxDeg; ++k) cf[k] = (l16 < W && k <= deg) ? a.kpoly[(long)k * W + l16] : T(0);
   T* ktb = ktbuf + 16 * (threadIdx.x >> 4);
+  // the polynomial of footprint cell l16 (zero for l16 >= W and above the degree), in registers
+  T cf[DM + 1];
+#pragma unroll
+  for (int k = 0; k <= DM; ++k) cf[k] = (l16 < W && k <= deg) ? a.kpoly[(long)k * W + l16] : T(0);
   // first cells of a binned point, relative to the tile; lr / lc outside the tile never happen
-  auto cells = [&](int p, int& lr, int& lc, double& ut, double& up, int& i0, int& j0u) {
+  auto cells = [&](int p, int r0, int& lr, int& lc, double& ut, double& up, int& i0, int& j0u) {
     const double2 tp = a.pts[p];
     ut = tp.x * scale;
     up = tp.y * scale;
@@ -714,16 +725,72 @@ __global__ __launch_bounds__(256) void k_spread_rows(SpreadArgs<T> a, RowsLayout
     lr = i0 - a.row_lo - r0;
     lc = j0 - c0;
   };
+  // ring rows [rb, re) of tile-row origin r0 (slot base) onto the fine grid, then cleared
+  auto flush_rows = [&](int r0, int base, int rb, int re) {
+    for (int i = threadIdx.x; i < (re - rb) * aw; i += blockDim.x) {
+      const int rr = rb + i / aw, cc = i % aw;
+      int slot = base + rr;  // base < R, rr < R
+      slot -= slot >= R ? R : 0;
+      double* cell = acc + 2 * (slot * aw + cc);
+      const double2 v = *reinterpret_cast<const double2*>(cell);
+      *reinterpret_cast<double2*>(cell) = double2{0.0, 0.0};
+      const int grow = r0 + rr;
+      if (grow >= a.nrow) continue;
+      const T vr = (T)v.x, vi = (T)v.y;
+      if (vr == T(0) && vi == T(0)) continue;
+      const int gc = (c0 + cc) % a.nf;  // a strip can be wider than a small grid
+      T* g = reinterpret_cast<T*>(a.grid + (long)grow * a.nf + gc);
+      if (grow >= excl_r0 && grow < excl_r1 && cc >= W - 1 && cc < excl_c1) {
+        *reinterpret_cast<c2_t<T>*>(g) = c2_t<T>{vr, vi};
+      } else {
+        atomicAdd(&g[0], vr);
+        atomicAdd(&g[1], vi);
+      }
+    }
+  };
+  for (int t = t0; t < t1; ++t) {
+  const int tb = t * a.ntc + tile_c;
+  const int s0 = a.tile_start[tb], s1 = a.tile_start[tb + 1];
+  const int r0 = t * tr;
+  const int base = ((t - t0) * tr) % R;  // ring slot of the tile's first row
   for (int cb = s0; cb < s1; cb += lay.cap) {
     const int n = min(lay.cap, s1 - cb);
     for (int i = threadIdx.x; i < nkey; i += blockDim.x) cnt[i] = 0;
     __syncthreads();
-    // pass 1: counts per (slice, row)
-    for (int q = threadIdx.x; q < n; q += blockDim.x) {
-      int lr, lc, i0, j0u;
-      double ut, up;
-      cells(cb + q, lr, lc, ut, up, i0, j0u);
-      if (lr >= 0 && lr < tr && lc >= 0 && lc < lay.tc) atomicAdd(&cnt[(lc / SW) * tr + lr], 1);
+    // pass 1: each thread holds up to kRowsPts points of the pass: its key (slice, row), polynomial arguments,
+    // column offset and (rotated) value, whose gathers are in flight across the count and the scan
+    int keyv[kRowsPts], offv[kRowsPts];
+    T2 uuv[kRowsPts];
+    double vrv[kRowsPts], viv[kRowsPts];
+#pragma unroll
+    for (int k = 0; k < kRowsPts; ++k) {
+      const int q = threadIdx.x + k * 256;
+      keyv[k] = -1;
+      offv[k] = 0;
+      uuv[k] = T2{T(0), T(0)};
+      vrv[k] = viv[k] = 0.0;
+      if (q < n) {
+        const int p = cb + q;
+        int lr, lc, i0, j0u;
+        double ut, up;
+        cells(p, r0, lr, lc, ut, up, i0, j0u);
+        if (lr >= 0 && lr < tr && lc >= 0 && lc < lay.tc) {
+          keyv[k] = (lc / SW) * tr + lr;
+          offv[k] = lc % SW;
+          uuv[k] = T2{(T)((i0 - ut) + 0.5 * W - 0.5), (T)((j0u - up) + 0.5 * W - 0.5)};
+          const int o = a.idx[p];
+          vrv[k] = a.in_re[o];
+          viv[k] = (a.spin && a.in_im) ? a.in_im[o] : 0.0;
+          if (a.spin && a.gamma) {
+            double sn, cs;
+            sincos(-a.spin * a.gamma[o], &sn, &cs);
+            const double xr = vrv[k] * cs - viv[k] * sn, xi = vrv[k] * sn + viv[k] * cs;
+            vrv[k] = xr;
+            viv[k] = xi;
+          }
+          atomicAdd(&cnt[keyv[k]], 1);
+        }
+      }
     }
     __syncthreads();
     // exclusive scan (one wave, nkey <= 16 x 32)
@@ -736,8 +803,8 @@ __global__ __launch_bounds__(256) void k_spread_rows(SpreadArgs<T> a, RowsLayout
       }
       int incl = sum;
       for (int d = 1; d < 64; d <<= 1) {
-        const int t = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += t;
+        const int u = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += u;
       }
       int run = incl - sum;
       for (int i = 0; i < per; ++i) {
@@ -751,27 +818,13 @@ __global__ __launch_bounds__(256) void k_spread_rows(SpreadArgs<T> a, RowsLayout
     }
     __syncthreads();
     // pass 2: records in sorted order (cnt counts down as the cursor)
-    for (int q = threadIdx.x; q < n; q += blockDim.x) {
-      const int p = cb + q;
-      int lr, lc, i0, j0u;
-      double ut, up;
-      cells(p, lr, lc, ut, up, i0, j0u);
-      if (!(lr >= 0 && lr < tr && lc >= 0 && lc < lay.tc)) continue;
-      const int key = (lc / SW) * tr + lr;
-      const int pos = start[key] + atomicSub(&cnt[key], 1) - 1;
-      const int o = a.idx[p];
-      double vr = a.in_re[o];
-      double vi = (a.spin && a.in_im) ? a.in_im[o] : 0.0;
-      if (a.spin && a.gamma) {
-        double sn, cs;
-        sincos(-a.spin * a.gamma[o], &sn, &cs);
-        const double xr = vr * cs - vi * sn, xi = vr * sn + vi * cs;
-        vr = xr;
-        vi = xi;
-      }
-      val[pos] = double2{vr, vi};
-      uu[pos] = T2{(T)((i0 - ut) + 0.5 * W - 0.5), (T)((j0u - up) + 0.5 * W - 0.5)};
-      offs[pos] = (unsigned char)(lc % SW);
+#pragma unroll
+    for (int k = 0; k < kRowsPts; ++k) {
+      if (keyv[k] < 0) continue;
+      const int pos = start[keyv[k]] + atomicSub(&cnt[keyv[k]], 1) - 1;
+      val[pos] = double2{vrv[k], viv[k]};
+      uu[pos] = uuv[k];
+      offs[pos] = (unsigned char)offv[k];
     }
     __syncthreads();
     // row walk: register window of W rows x (re, im), slot of row r = r mod W
@@ -792,7 +845,7 @@ __global__ __launch_bounds__(256) void k_spread_rows(SpreadArgs<T> a, RowsLayout
               const T2 u = uu[q];
               const int off = offs[q];
               T kt, kp;
-              horner_any(cf, deg, u.x, u.y, kt, kp);
+              horner_any<DM>(cf, deg, u.x, u.y, kt, kp);
               // the row weights reach the group's lanes through its LDS slot (LDS operations of one wave
               // complete in order; the fences keep the compiler from moving the reads above the write)
               ktb[l16] = kt;
@@ -815,10 +868,12 @@ __global__ __launch_bounds__(256) void k_spread_rows(SpreadArgs<T> a, RowsLayout
             }
           }
         }
-        if (r < tr + W - 1) {  // row r is complete: add it to the tile and reuse the slot for row r + W
+        if (r < tr + W - 1) {  // row r is complete: add it to the ring and reuse the slot for row r + W
           if (ar[K] != 0.0 || ai[K] != 0.0) {
-            atomicAdd(&acol[2 * r * aw], ar[K]);
-            atomicAdd(&acol[2 * r * aw + 1], ai[K]);
+            int slot = base + r;
+            slot -= slot >= R ? R : 0;
+            atomicAdd(&acol[2 * slot * aw], ar[K]);
+            atomicAdd(&acol[2 * slot * aw + 1], ai[K]);
           }
           ar[K] = 0.0;
           ai[K] = 0.0;
@@ -827,18 +882,12 @@ __global__ __launch_bounds__(256) void k_spread_rows(SpreadArgs<T> a, RowsLayout
     }
     __syncthreads();  // records are rewritten by the next pass
   }
-  // the tile (+ halo) onto the fine grid
-  for (int i = threadIdx.x; i < nacc; i += blockDim.x) {
-    const int rr = i / aw, cc = i - rr * aw;
-    const int grow = r0 + rr;
-    if (grow >= a.nrow) continue;
-    const T vr = (T)acc[2 * i], vi = (T)acc[2 * i + 1];
-    if (vr == T(0) && vi == T(0)) continue;
-    const int gc = (c0 + cc) % a.nf;  // a tile can be wider than a small grid
-    T* g = reinterpret_cast<T*>(a.grid + (long)grow * a.nf + gc);
-    atomicAdd(&g[0], vr);
-    atomicAdd(&g[1], vi);
+  // rows r0 .. r0 + tr - 1 get nothing from later tiles of the strip
+  flush_rows(r0, base, 0, tr);
+  __syncthreads();
   }
+  // the W - 1 rows below the last tile are shared with the next segment
+  flush_rows((t1 - 1) * tr, ((t1 - 1 - t0) * tr) % R, tr, R);
 }
 
 template <typename T>
@@ -1062,9 +1111,11 @@ void launch_spread_rows(int w, dim3 grid, hipStream_t st, const SpreadArgs<T>& a
     throw Error{kErrUnsupported, "nufft: no register row-window spreading for this width"};
   } else {
     if (w == W) {
-      NUSHT_HIP(hipFuncSetAttribute((const void*)k_spread_rows<T, W>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lay.bytes));
-      hipLaunchKernelGGL((k_spread_rows<T, W>), grid, dim3(256), lay.bytes, st, a, lay);
+      // polynomial coefficients in registers: 13 for degrees up to 12 (eps >= 1e-8), 25 above
+      const void* k = lay.deg <= 12 ? (const void*)k_spread_rows<T, W, 12> : (const void*)k_spread_rows<T, W, 24>;
+      NUSHT_HIP(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lay.bytes));
+      if (lay.deg <= 12) hipLaunchKernelGGL((k_spread_rows<T, W, 12>), grid, dim3(256), lay.bytes, st, a, lay);
+      else hipLaunchKernelGGL((k_spread_rows<T, W, 24>), grid, dim3(256), lay.bytes, st, a, lay);
     } else {
       launch_spread_rows<T, W + 1>(w, grid, st, a, lay);
     }
@@ -1238,20 +1289,27 @@ NufftPlan::NufftPlan(const NufftParams& p, long npts_max, hipStream_t st, int ti
     tr = std::max(1, std::min(32, tr));
     tile_r_ = tr;
     tile_c_ = kRowsGroups * (17 - p.w);
-    // LDS: the accumulation tile plus point records, for two workgroups per CU when 512 or more records fit
-    // beside the tile in 78 KiB, else one workgroup with up to 156 KiB
+    // LDS: the accumulation tile plus point records (value, row and column arguments, column offset): three
+    // workgroups per CU when 256 records fit in 52 KiB, two when they fit in 78 KiB, else one with up to
+    // 156 KiB; the records left over in the chosen budget are used too (fewer passes over a tile)
     const size_t rec = sizeof(double2) + (p.single ? sizeof(float2) : sizeof(double2)) + 1;
     const RowsLayout base = p.single ? rows_layout<float>(p.w, tr, p_.poly_deg, 0)
                                      : rows_layout<double>(p.w, tr, p_.poly_deg, 0);
-    const size_t budget = base.bytes + 512 * rec <= 78 * 1024 ? 78 * 1024 : 156 * 1024;
+    const size_t budget = base.bytes + 256 * rec <= 52 * 1024   ? 52 * 1024
+                          : base.bytes + 256 * rec <= 78 * 1024 ? 78 * 1024
+                                                                : 156 * 1024;
     NUSHT_REQUIRE(base.bytes + 64 * rec <= budget, kErrUnsupported, "nufft: spreading tile too large for LDS");
     long cap = (long)((budget - base.bytes) / rec);
-    cap = std::max(64L, std::min(2048L, cap - cap % 64));
+    cap = std::max(64L, std::min(256L * kRowsPts, cap - cap % 64));
     rows_cap_ = (int)cap;
   }
   ntile_r_ = ceil_div(p.nrow, tile_r_);
   ntile_c_ = ceil_div(p.nf, tile_c_);
   const int ntiles = ntile_r_ * ntile_c_;
+  // the row-window spreading walks up to 16 tiles down a column strip per workgroup, keeping about 3000
+  // workgroups (NUSHT_SPREAD_SEG sets the count)
+  rows_seg_ = std::max(1, std::min(16, ntiles / 3072));
+  if (const char* e = getenv("NUSHT_SPREAD_SEG"); e && e[0]) rows_seg_ = std::max(1, std::min(ntile_r_, atoi(e)));
   npts_max_ = std::max<long>(npts_max, 1);
   NUSHT_REQUIRE(npts_max_ < (1L << 31), kErrUnsupported, "nufft: too many points");
   pts_.alloc(sizeof(double2) * npts_max_);
@@ -1376,8 +1434,10 @@ void NufftPlan::type1_impl(const double* ire, const double* iim, const double* g
                      p.poly_deg, pts_.as<double2>(), idx_.as<int>(), tile_start_.as<int>(), ire, iim, gamma,
                      grid_.as<T2>()};
     if (rows_spread_) {
-      const RowsLayout lay = rows_layout<T>(p.w, tile_r_, p.poly_deg, rows_cap_);
-      launch_spread_rows<T, kRowsMinW>(p.w, dim3(ntile_r_ * ntile_c_), st, sa, lay);
+      RowsLayout lay = rows_layout<T>(p.w, tile_r_, p.poly_deg, rows_cap_);
+      lay.seg = rows_seg_;
+      lay.ntr = ntile_r_;
+      launch_spread_rows<T, kRowsMinW>(p.w, dim3(ntile_c_ * (int)ceil_div(ntile_r_, rows_seg_)), st, sa, lay);
     } else {
       const size_t lds = sizeof(double2) * (size_t)(tile_r_ + p.w) * (tile_c_ + p.w);  // FP64 accumulation tile
       launch_spread<T, 2>(p.w, dim3(ntile_r_ * ntile_c_), lds, st, sa);

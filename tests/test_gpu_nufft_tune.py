@@ -7,6 +7,9 @@
   row-window spreading (1 .. 32 rows) gives the direct adjoint.
 * The row-window spreading equals the LDS-atomic spreading (NUSHT_SPREAD_ROWS=0) to 1e-12, also when tiles
   hold more points than one pass of the kernel's LDS records (many points on a small grid).
+* Workgroups that walk several tiles down a column strip (NUSHT_SPREAD_SEG, carried halo rows, plain stores
+  for cells no other workgroup touches) give the direct adjoint for every segment length and tile height,
+  including one segment for the whole strip.
 """
 import numpy as np
 import pytest
@@ -128,3 +131,24 @@ def test_spread_rows_equals_lds_atomic_kernel(eps, spin, sigma, L, npts, monkeyp
         del T
     tol = 1e-12 if eps < 1e-6 else 1e-5  # FP32 grid: the global flush adds FP32 values in a varying order
     assert _relmax(outs[0], outs[1]) < tol
+
+
+@pytest.mark.parametrize("seg,rows", [(2, 8), (3, 1), (5, 4), (7, 16), (1000, 8), (1000, 32)])
+@pytest.mark.parametrize("eps,spin", [(1e-7, 2), (1e-10, 0), (1e-5, 2)])
+def test_spread_strip_segments(seg, rows, eps, spin, monkeypatch):
+    monkeypatch.setenv("NUSHT_SPREAD_SEG", str(seg))
+    monkeypatch.setenv("NUSHT_SPREAD_TR", str(rows))
+    L = 64
+    rng = np.random.default_rng(840 + seg + rows)
+    th, ph = _points(rng, 6000)
+    th[:2] = [0.0, np.pi]
+    T = _T(L, eps, spin)
+    print(f"case spread strip segments seg={seg} rows={rows} eps={eps} spin={spin}", flush=True)
+    x = rng.standard_normal(th.size) + (1j * rng.standard_normal(th.size) if spin else 0)
+    xin = _t(x.real if spin == 0 else np.array([x.real, x.imag]))
+    ad = T.nusht2d1(L, L, xin, _t(np.stack([th, ph], 1)), spin=spin).cpu().numpy()
+    info = next(iter(T._plans.values())).info()
+    assert _rows_kernel_used(info) and info["nufft_tile_rows"] == rows
+    refad = osht.adjoint_direct(x, L, spin, th, ph)
+    tol = 30 * eps if eps >= 1e-6 else 10 * eps
+    assert _relmax(ad.reshape(refad.shape), refad) < tol
