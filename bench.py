@@ -580,7 +580,8 @@ def main():
     if interp:
         w_i, _ = stage_work("nufft_interp", info, L, npix)
         a = w_i / (interp[0] / interp[1] / 1e3) / 1e9
-        nufft_roof = {"achieved_GBs": a, "frac": a / PEAK_HBM_GBS, "avg_ms": interp[0] / interp[1]}
+        nufft_roof = {"achieved_GBs": a, "frac": a / PEAK_HBM_GBS, "avg_ms": interp[0] / interp[1],
+                      "traffic": profiled_traffic("nufft_interp", 1)}
 
     own = own_deflection_batch(args, L, eps, gclms, dlms, geominfo)
     print(f"[bench] batched, own deflection per map: {own['value']:.2f} maps/s", file=sys.stderr, flush=True)
