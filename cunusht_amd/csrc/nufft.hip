@@ -59,63 +59,84 @@ struct ThetaArgs {
   c2_t<T>* Tout;  // [nrow][nm]
 };
 
+// Two columns per workgroup.  The doubled sequence of column m is even or odd (x_{-n} = +-x_n, sign (-1)^(s+m)),
+// and rows 2p, 2p + 1 have opposite signs; so one transform pair serves both: with X = XA + XB and
+// XA_{-k} = sA XA_k, XB_{-k} = -sA XB_k, the deconvolved spectrum d_k (fA XA_k + fB XB_k) is
+// d_k (cp X_k + cm X_{-k}), cp = (fA + fB) / 2, cm = sA (fA - fB) / 2, and the inverse transform T = TA + TB
+// separates by TA_t = (T_t + sA T_{-t}) / 2, TB_t = (T_t - sA T_{-t}) / 2.
 template <typename T, int NT, int NMAX>
 __global__ __launch_bounds__(NT) void k_theta2(ThetaArgs<T> a) {
   using T2 = c2_t<T>;
   extern __shared__ unsigned char smem_raw[];
   T2* buf = reinterpret_cast<T2*>(smem_raw);
-  const int row = xcd_remap(blockIdx.x, gridDim.x);
-  const int m = a.spin ? row - a.lmax : row;
-  const int am = m < 0 ? -m : m;
-  const bool neg = ((a.spin + am) & 1) != 0;
-  const double2* g = a.G + (long)row * a.gsm;
-  for (int r = threadIdx.x; r < a.ntd; r += NT) {
-    double2 v;
-    if (r < a.ntheta) v = g[r];
-    else { v = g[a.ntd - r]; if (neg) v = double2{-v.x, -v.y}; }
-    buf[r] = T2{(T)v.x, (T)v.y};
+  const int rowA = 2 * xcd_remap(blockIdx.x, gridDim.x), rowB = rowA + 1;
+  const bool hasB = rowB < a.nmrows;
+  const int mA = a.spin ? rowA - a.lmax : rowA, mB = mA + 1;
+  const int amA = mA < 0 ? -mA : mA, amB = mB < 0 ? -mB : mB;
+  const double sA = ((a.spin + amA) & 1) ? -1.0 : 1.0;
+  const double2* gA = a.G + (long)rowA * a.gsm;
+  const double2* gB = a.G + (long)rowB * a.gsm;
+  // ring r and its mirror ntd - r (ntheta - 1 = ntd / 2 rings apart from the poles)
+  for (int r = threadIdx.x; r < a.ntheta; r += NT) {
+    double2 u = gA[r], w{0.0, 0.0};
+    if (hasB) w = gB[r];
+    buf[r] = T2{(T)(u.x + w.x), (T)(u.y + w.y)};
+    if (r > 0 && r < a.ntheta - 1) buf[a.ntd - r] = T2{(T)(sA * (u.x - w.x)), (T)(sA * (u.y - w.y))};
   }
   __syncthreads();
   fft_lds<T, NT, NMAX, false, true, std::is_same<T, float>::value>(buf, a.ftd);
-  // deconvolve, pad to nf (conjugated for the inverse transform)
+  // split the parities, deconvolve, pad to nf (conjugated for the inverse transform)
   constexpr int E = (NMAX + NT - 1) / NT;
   T2 v[E];
-  const double fm = a.deconv[am] / a.ntd;
+  const double fA = a.deconv[amA] / a.ntd, fB = hasB ? a.deconv[amB] / a.ntd : 0.0;
+  const double cp = 0.5 * (fA + fB), cm = 0.5 * sA * (fA - fB);
 #pragma unroll
   for (int i = 0; i < E; ++i) {
-    int k = threadIdx.x + i * NT;
+    const int k = threadIdx.x + i * NT;
     v[i] = T2{T(0), T(0)};
     if (k < a.nf) {
+      int kk = -1, r = 0, rm = 0;  // |frequency|, its buffer index and the mirrored one
       if (k <= a.lmax) {
-        T2 s = buf[k];
-        T f = (T)(fm * a.deconv[k]);
-        v[i] = T2{s.x * f, -s.y * f};
+        kk = k;
+        r = k;
+        rm = k == 0 ? 0 : a.ntd - k;
       } else if (k >= a.nf - a.lmax) {
-        int kk = a.nf - k;
-        T2 s = buf[a.ntd - kk];
-        T f = (T)(fm * a.deconv[kk]);
-        v[i] = T2{s.x * f, -s.y * f};
+        kk = a.nf - k;
+        r = a.ntd - kk;
+        rm = kk;
+      }
+      if (kk >= 0) {
+        const T2 s = buf[r], u = buf[rm];
+        const double d = a.deconv[kk];
+        const T fp = (T)(cp * d), fn = (T)(cm * d);
+        v[i] = T2{fp * s.x + fn * u.x, -(fp * s.y + fn * u.y)};
       }
     }
   }
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < E; ++i) {
-    int k = threadIdx.x + i * NT;
+    const int k = threadIdx.x + i * NT;
     if (k < a.nf) buf[k] = v[i];
   }
   __syncthreads();
   fft_lds<T, NT, NMAX, false, true, std::is_same<T, float>::value>(buf, a.fnf);
-  // rows of the kept band; spin 0 also fills the conjugate column
+  // rows of the kept band, both columns; spin 0 also fills the conjugate columns
+  const T hs = (T)(0.5 * sA);
   for (int q = threadIdx.x; q < a.nrow; q += NT) {
     int t = a.row_lo + q;
     t %= a.nf;
     if (t < 0) t += a.nf;
-    T2 s = buf[t];
-    T2 val{s.x, -s.y};
+    const T2 s = buf[t], u = buf[t == 0 ? 0 : a.nf - t];
+    const T2 valA{T(0.5) * s.x + hs * u.x, -(T(0.5) * s.y + hs * u.y)};
     T2* orow = a.Tout + (long)q * a.nm;
-    orow[m + a.lmax] = val;
-    if (!a.spin && m > 0) orow[a.lmax - m] = T2{val.x, -val.y};
+    orow[mA + a.lmax] = valA;
+    if (!a.spin && mA > 0) orow[a.lmax - mA] = T2{valA.x, -valA.y};
+    if (hasB) {
+      const T2 valB{T(0.5) * s.x - hs * u.x, -(T(0.5) * s.y - hs * u.y)};
+      orow[mB + a.lmax] = valB;
+      if (!a.spin) orow[a.lmax - mB] = T2{valB.x, -valB.y};
+    }
   }
 }
 
@@ -170,64 +191,90 @@ __global__ __launch_bounds__(NT) void k_phi1(PhiArgs<T> a) {
 }
 
 // ------------------------------------------------------------------ theta pass, type 1 (adjoint)
+// Exact transpose of k_theta2, two columns per workgroup: the kept rows of both columns enter the nf-point
+// buffer through the transposed parity split (u_t = (uA_t + sA uA_{-t}) / 2 + (uB_t - sA uB_{-t}) / 2), the
+// spectrum through the transposed deconvolution d_k (cp Y_k + cm Y_{-k}), and the two columns leave through
+// their own foldings (signs sA and -sA).
 template <typename T, int NT, int NMAX>
 __global__ __launch_bounds__(NT) void k_theta1(ThetaArgs<T> a) {
   using T2 = c2_t<T>;
   extern __shared__ unsigned char smem_raw[];
   T2* buf = reinterpret_cast<T2*>(smem_raw);
-  const int row = xcd_remap(blockIdx.x, gridDim.x);
-  const int m = a.spin ? row - a.lmax : row;
-  const int am = m < 0 ? -m : m;
-  const bool neg = ((a.spin + am) & 1) != 0;
-  for (int k = threadIdx.x; k < a.nf; k += NT) buf[k] = T2{T(0), T(0)};
-  __syncthreads();
-  for (int q = threadIdx.x; q < a.nrow; q += NT) {
-    int t = a.row_lo + q;
-    t %= a.nf;
-    if (t < 0) t += a.nf;
-    buf[t] = a.Tout[(long)q * a.nm + m + a.lmax];
+  const int rowA = 2 * xcd_remap(blockIdx.x, gridDim.x), rowB = rowA + 1;
+  const bool hasB = rowB < a.nmrows;
+  const int mA = a.spin ? rowA - a.lmax : rowA, mB = mA + 1;
+  const int amA = mA < 0 ? -mA : mA, amB = mB < 0 ? -mB : mB;
+  const double sA = ((a.spin + amA) & 1) ? -1.0 : 1.0;
+  const T hs = (T)(0.5 * sA);
+  const T2* colA = a.Tout + mA + a.lmax;
+  const T2* colB = a.Tout + mB + a.lmax;
+  // positions k and -k (the same for k = 0 and nf / 2) from the kept rows there, if any
+  for (int k = threadIdx.x; k <= a.nf / 2; k += NT) {
+    const int km = k == 0 ? 0 : a.nf - k;
+    int q = k - a.row_lo;
+    if (q >= a.nf) q -= a.nf;
+    int qm = km - a.row_lo;
+    if (qm >= a.nf) qm -= a.nf;
+    const T2 zero{T(0), T(0)};
+    const T2 uA = q < a.nrow ? colA[(long)q * a.nm] : zero;
+    const T2 uAm = qm < a.nrow ? colA[(long)qm * a.nm] : zero;
+    const T2 uB = hasB && q < a.nrow ? colB[(long)q * a.nm] : zero;
+    const T2 uBm = hasB && qm < a.nrow ? colB[(long)qm * a.nm] : zero;
+    buf[k] = T2{T(0.5) * (uA.x + uB.x) + hs * (uAm.x - uBm.x), T(0.5) * (uA.y + uB.y) + hs * (uAm.y - uBm.y)};
+    if (km != k)
+      buf[km] = T2{T(0.5) * (uAm.x + uBm.x) + hs * (uA.x - uB.x), T(0.5) * (uAm.y + uBm.y) + hs * (uA.y - uB.y)};
   }
   __syncthreads();
   fft_lds<T, NT, NMAX, false, true, std::is_same<T, float>::value>(buf, a.fnf);
   constexpr int E = (NMAX + NT - 1) / NT;
   T2 v[E];
-  const double fm = a.deconv[am] / a.ntd;
+  const double fA = a.deconv[amA] / a.ntd, fB = hasB ? a.deconv[amB] / a.ntd : 0.0;
+  const double cp = 0.5 * (fA + fB), cm = 0.5 * sA * (fA - fB);
 #pragma unroll
   for (int i = 0; i < E; ++i) {
-    int r = threadIdx.x + i * NT;
+    const int r = threadIdx.x + i * NT;
     v[i] = T2{T(0), T(0)};
     if (r < a.ntd) {
+      int kk = -1, pos = 0, posm = 0;  // |frequency|, its nf-buffer position and the mirrored one
       if (r <= a.lmax) {
-        T2 s = buf[r];
-        T f = (T)(fm * a.deconv[r]);
-        v[i] = T2{s.x * f, -s.y * f};
+        kk = r;
+        pos = r;
+        posm = r == 0 ? 0 : a.nf - r;
       } else if (r >= a.ntd - a.lmax) {
-        int kk = a.ntd - r;
-        T2 s = buf[a.nf - kk];
-        T f = (T)(fm * a.deconv[kk]);
-        v[i] = T2{s.x * f, -s.y * f};
+        kk = a.ntd - r;
+        pos = a.nf - kk;
+        posm = kk;
+      }
+      if (kk >= 0) {
+        const T2 s = buf[pos], u = buf[posm];
+        const double d = a.deconv[kk];
+        const T fp = (T)(cp * d), fn = (T)(cm * d);
+        v[i] = T2{fp * s.x + fn * u.x, -(fp * s.y + fn * u.y)};
       }
     }
   }
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < E; ++i) {
-    int r = threadIdx.x + i * NT;
+    const int r = threadIdx.x + i * NT;
     if (r < a.ntd) buf[r] = v[i];
   }
   __syncthreads();
   fft_lds<T, NT, NMAX, false, true, std::is_same<T, float>::value>(buf, a.ftd);
-  // fold back onto the ntheta rings (conjugate back from the inverse trick)
-  double2* g = const_cast<double2*>(a.G) + (long)row * a.gsm;
+  // fold back onto the ntheta rings (conjugate back from the inverse trick), both columns
+  double2* gA = const_cast<double2*>(a.G) + (long)rowA * a.gsm;
+  double2* gB = const_cast<double2*>(a.G) + (long)rowB * a.gsm;
   for (int j = threadIdx.x; j < a.ntheta; j += NT) {
-    T2 s = buf[j];
-    double re = (double)s.x, im = -(double)s.y;
+    const T2 s = buf[j];
+    const double re = (double)s.x, im = -(double)s.y;
+    double ur = 0.0, ui = 0.0;
     if (j > 0 && j < a.ntheta - 1) {
-      T2 u = buf[a.ntd - j];
-      double ur = (double)u.x, ui = -(double)u.y;
-      if (neg) { re -= ur; im -= ui; } else { re += ur; im += ui; }
+      const T2 u = buf[a.ntd - j];
+      ur = sA * (double)u.x;
+      ui = -sA * (double)u.y;
     }
-    g[j] = double2{re, im};
+    gA[j] = double2{re + ur, im + ui};
+    if (hasB) gB[j] = double2{re - ur, im - ui};
   }
 }
 
@@ -1396,7 +1443,7 @@ void NufftPlan::type2_impl(const double2* G, long sm, const double* gamma, doubl
   const size_t lds = sizeof(T2) * (size_t)std::max(p.ntd, p.nf);
   {
     StageScope sc("theta_pass_t2", st);
-    NUSHT_LAUNCH_SIZED(k_theta2, T, std::max(p.ntd, p.nf), nmrows, lds, ta, st);
+    NUSHT_LAUNCH_SIZED(k_theta2, T, std::max(p.ntd, p.nf), (nmrows + 1) / 2, lds, ta, st);  // column pairs
   }
   PhiArgs<T> pa{p.lmax, p.nf, p.nm, p.nrow, fnf, T_.as<T2>(), grid_.as<T2>()};
   {
@@ -1454,7 +1501,7 @@ void NufftPlan::type1_impl(const double* ire, const double* iim, const double* g
                   deconv_.as<double>(), ftd, fnf, T_.as<T2>()};
   const size_t lds = sizeof(T2) * (size_t)std::max(p.ntd, p.nf);
   StageScope sc("theta_pass_t1", st);
-  NUSHT_LAUNCH_SIZED(k_theta1, T, std::max(p.ntd, p.nf), nmrows, lds, ta, st);
+  NUSHT_LAUNCH_SIZED(k_theta1, T, std::max(p.ntd, p.nf), (nmrows + 1) / 2, lds, ta, st);  // column pairs
 }
 
 void NufftPlan::type2(const double2* G, long sm, const double* gamma, double* ore, double* oim, hipStream_t st) {
