@@ -636,13 +636,15 @@ constexpr int kMaxUnrolledW = 16;
 // W-column footprint of every point whose first column lies in the group's slice [g SW, g SW + SW).  The
 // points of the tile are counting-sorted in LDS by (slice, first row); each group then walks the rows
 // r = 0, 1, ... of its slice, adding every point of row r into W rows of register accumulators (lane j:
-// column g SW + j; kernel row weights by DPP row broadcasts, the column weight shuffled in by the point's
-// column offset).  Row r receives nothing after the points of row r, so its accumulator is added to the
-// LDS tile once (16 lanes, one ds_add_f64 pair) and reused as row r + W: a rotating window whose slot
-// index r mod W is wave-uniform (the four groups of a wave take slices w, w + 4, w + 8, w + 12 and step
-// through the rows together), so every register index is a compile-time constant of the W-fold unrolled
-// row loop.  LDS atomics: two per fine cell and covering group instead of 2 W^2 per point.  Tiles with more
-// points than the LDS record buffer holds are spread in several passes.
+// column g SW + j; the kernel row weights written by the group into an LDS slot and read back by all its
+// lanes, the column weight shuffled in by the point's column offset).  Row r receives nothing after the
+// points of row r, so its accumulator is added to the LDS accumulation rows once (16 lanes, one ds_add_f64
+// pair) and reused as row r + W: a rotating window whose slot index r mod W is wave-uniform (the four groups
+// of a wave take slices w, w + 4, w + 8, w + 12 and step through the rows together), so every register index
+// is a compile-time constant of the W-fold unrolled row loop.  LDS atomics: two per fine cell and covering
+// group instead of 2 W^2 per point.  Tiles with more points than the LDS record buffer holds are spread in
+// several passes.  A workgroup walks several tiles down one column strip and keeps the accumulation rows in a
+// ring, so the W - 1 halo rows below a tile carry over to the next instead of going to HBM with atomics.
 constexpr int kRowsMinW = 8;
 constexpr int kRowsMaxW = 16;
 constexpr int kRowsGroups = 16;  // 16-lane groups per 256-thread workgroup (= slices per tile)
