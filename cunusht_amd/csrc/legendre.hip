@@ -376,8 +376,16 @@ __global__ __launch_bounds__(64 * WPB) void k_synth(SynthArgs a) {
     cur = nxt;
     lc += CH;
   }
-  // fast phase
-  while (lc + CH - 1 <= L) {
+  // fast phase, two chunks per iteration so that the two buffers alternate without copies (the loads reach at
+  // most CH entries past lmax, as in the single-chunk step)
+  while (lc + 2 * CH - 1 <= L) {
+    load(lc + CH, nxt);
+    chunk(lc, cur, F_{}, T_{});
+    load(lc + 2 * CH, cur);
+    chunk(lc + CH, nxt, F_{}, T_{});
+    lc += 2 * CH;
+  }
+  if (lc + CH - 1 <= L) {
     load(lc + CH, nxt);
     chunk(lc, cur, F_{}, T_{});
     cur = nxt;
@@ -614,7 +622,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     cur = nxt;
     lc += 4;
   }
-  while (lc + 3 <= L) {
+  // two chunks per iteration, so the two buffers alternate without copies (loads reach at most one 4-l block
+  // past lmax, as the single-chunk step does)
+  while (lc + 7 <= L) {
+    load(lc + 4, nxt);
+    chunk(lc, cur, F_{}, T_{});
+    load(lc + 8, cur);
+    chunk(lc + 4, nxt, F_{}, T_{});
+    lc += 8;
+  }
+  if (lc + 3 <= L) {
     load(lc + 4, nxt);
     chunk(lc, cur, F_{}, T_{});
     cur = nxt;
