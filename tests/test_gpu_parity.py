@@ -335,6 +335,39 @@ def test_nusht2d1_adjointness(spin, eps):
     assert _rel2(ad, ref) < 10 * max(eps, 1e-12)
 
 
+@pytest.mark.parametrize("spin", [1, 0, 2])
+@pytest.mark.parametrize("L", [33, 34])
+@pytest.mark.parametrize("eps", [1e-10, 1e-5])
+def test_nusht_pair_odd_spins_and_lmax(spin, L, eps):
+    """nusht2d2 vs direct summation and nusht2d1 vs the direct adjoint at spin 1 (the parity (s + m) of the doubled
+    theta columns flips with s, and the theta pass transforms columns of opposite parity together), spin 0 and
+    spin 2, at odd and even lmax (spin 0 then has an even or odd number of m columns; with spin, 2 lmax + 1
+    columns leave one unpaired)."""
+    import cunusht_amd
+    rng = np.random.default_rng(900 + 10 * spin + L + int(-np.log10(eps)))
+    npts = 2500
+    theta = np.arccos(rng.uniform(-1, 1, npts))
+    phi = rng.uniform(0, 2 * np.pi, npts)
+    theta[:2] = [0.0, np.pi]
+    loc = np.stack([theta, phi], 1)
+    nc = 1 if spin == 0 else 2
+    a = almutil.random_alm(L, spin, nc, rng)
+    ain = a[0] if spin == 0 else a
+    print(f"case pair spin={spin} L={L} eps={eps}", flush=True)
+    T = cunusht_amd.get_transformer("GPU")(("gl", {"lmax": L}), nuFFTtype=2, epsilon=eps, spin=spin)
+    got = T.nusht2d2(L, L, _t(ain), _t(loc), spin=spin).cpu().numpy()
+    ref = osht.synth_direct(ain, L, spin, theta, phi)
+    if spin:
+        ref = np.array([ref.real, ref.imag])
+    assert _rel2(got, ref) < 10 * max(eps, 1e-13)
+    x = rng.standard_normal((nc, npts))
+    xin = x[0] if spin == 0 else x
+    ad = T.nusht2d1(L, L, _t(xin), _t(loc), spin=spin).cpu().numpy().reshape(nc, -1)
+    vals = xin if spin == 0 else (x[0] + 1j * x[1])
+    aref = osht.adjoint_direct(vals, L, spin, theta, phi).reshape(nc, -1)
+    assert _rel2(ad, aref) < 30 * max(eps, 1e-13)
+
+
 # ------------------------------------------------------------------ lensing
 @pytest.mark.parametrize("spin", [0, 2])
 def test_gclm2lenmap_vs_oracle(spin):
