@@ -86,6 +86,7 @@ __global__ __launch_bounds__(kRingNT) void k_ring(RingArgs a) {
     __syncthreads();
     dft_lds<METHOD, double>(buf, Ld);
     const double th = a.theta[ring];
+    const double sth = sin(th), cth = cos(th);  // constant along the ring
     for (int k = threadIdx.x; k < n; k += kRingNT) {
       double2 f = cconj(buf[k]);
       if (a.mode == 0) {
@@ -95,7 +96,7 @@ __global__ __launch_bounds__(kRingNT) void k_ring(RingArgs a) {
         a.o1[ofs + k] = f.y;
       } else {
         double tp, pp;
-        deflect_point(th, ph0 + k * (2.0 * M_PI / n), f.x, f.y, tp, pp, a.o2 ? &a.o2[ofs + k] : nullptr);
+        deflect_point_sc(sth, cth, ph0 + k * (2.0 * M_PI / n), f.x, f.y, tp, pp, a.o2 ? &a.o2[ofs + k] : nullptr);
         a.o0[ofs + k] = tp;
         a.o1[ofs + k] = pp;
       }

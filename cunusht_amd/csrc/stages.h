@@ -11,8 +11,17 @@ namespace nusht {
 //   n3 = cos(t) cos(a) - sin(t) d_t sin(a)/a;  t' = atan2(sqrt(n1^2+n2^2), n3);
 //   p' = p - atan2(n2, n1), reduced by 2 pi only when >= 2 pi;
 //   gamma = atan2(d_p, d_t) - atan2(d_p, a sin(a) cot(t) + d_t cos(a)).
+// The same with the colatitude's sine and cosine given (constant along a ring).
+__host__ __device__ inline void deflect_point_sc(double st, double ct, double p, double dt, double dp, double& tp,
+                                                 double& pp, double* gamma);
+
 __host__ __device__ inline void deflect_point(double t, double p, double dt, double dp, double& tp, double& pp,
                                               double* gamma) {
+  deflect_point_sc(sin(t), cos(t), p, dt, dp, tp, pp, gamma);
+}
+
+__host__ __device__ inline void deflect_point_sc(double st, double ct, double p, double dt, double dp, double& tp,
+                                                 double& pp, double* gamma) {
   const double a2 = dt * dt + dp * dp;
   const double a = sqrt(a2);
   double sa, ca;
@@ -23,7 +32,6 @@ __host__ __device__ inline void deflect_point(double t, double p, double dt, dou
     sa = sin(a) / a;
     ca = cos(a);
   }
-  const double st = sin(t), ct = cos(t);
   const double n1 = st * ca + ct * dt * sa;
   const double n2 = -dp * sa;
   const double n3 = ct * ca - st * dt * sa;
