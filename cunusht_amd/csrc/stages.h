@@ -39,8 +39,11 @@ __host__ __device__ inline void deflect_point_sc(double st, double ct, double p,
   double q = p - atan2(n2, n1);
   pp = (q >= 2.0 * M_PI) ? q - 2.0 * M_PI : q;
   if (gamma) {
-    // a sin(a) = a^2 * sin(a)/a
-    *gamma = atan2(dp, dt) - atan2(dp, a2 * sa * ct / st + dt * ca);
+    // a sin(a) = a^2 * sin(a)/a.  Both angles have the ordinate dp, so they lie in one half-plane and their
+    // difference is the principal value atan2(dp (x - dt), dt x + dp^2) (signed zeros included); at the poles
+    // (cot t infinite) the two terms are kept
+    const double x = a2 * sa * ct / st + dt * ca;
+    *gamma = st != 0.0 ? atan2(dp * (x - dt), dt * x + dp * dp) : atan2(dp, dt) - atan2(dp, x);
   }
 }
 
