@@ -156,11 +156,27 @@ __global__ __launch_bounds__(NT) void k_phi2(PhiArgs<T> a) {
   T2* buf = reinterpret_cast<T2*>(smem_raw);
   const int q = blockIdx.x;
   const T2* trow = a.Tbuf + (long)q * a.nm;
-  for (int k = threadIdx.x; k < a.nf; k += NT) {
-    int m = k <= a.lmax ? k : (k >= a.nf - a.lmax ? k - a.nf : 0x7fffffff);
-    T2 v{T(0), T(0)};
-    if (m != 0x7fffffff) { T2 s = trow[m + a.lmax]; v = T2{s.x, -s.y}; }
-    buf[k] = v;
+  // Every load of the thread is issued before the first LDS write (nf <= NMAX): the loads are unconditional,
+  // from clamped in-range addresses, and positions outside the band are zeroed afterwards.
+  constexpr int E = (NMAX + NT - 1) / NT;
+  T2 v[E];
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int k = threadIdx.x + i * NT;
+    const bool in = k <= a.lmax || (k < a.nf && k >= a.nf - a.lmax);
+    const int m = k <= a.lmax ? k : k - a.nf;
+    v[i] = trow[in ? m + a.lmax : 0];
+  }
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int k = threadIdx.x + i * NT;
+    const bool in = k <= a.lmax || (k < a.nf && k >= a.nf - a.lmax);
+    v[i] = in ? T2{v[i].x, -v[i].y} : T2{T(0), T(0)};
+  }
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    const int k = threadIdx.x + i * NT;
+    if (k < a.nf) buf[k] = v[i];
   }
   __syncthreads();
   fft_lds<T, NT, NMAX, false, true, std::is_same<T, float>::value>(buf, a.fnf);
@@ -208,21 +224,54 @@ __global__ __launch_bounds__(NT) void k_theta1(ThetaArgs<T> a) {
   const T hs = (T)(0.5 * sA);
   const T2* colA = a.Tout + mA + a.lmax;
   const T2* colB = a.Tout + mB + a.lmax;
-  // positions k and -k (the same for k = 0 and nf / 2) from the kept rows there, if any
-  for (int k = threadIdx.x; k <= a.nf / 2; k += NT) {
-    const int km = k == 0 ? 0 : a.nf - k;
-    int q = k - a.row_lo;
-    if (q >= a.nf) q -= a.nf;
-    int qm = km - a.row_lo;
-    if (qm >= a.nf) qm -= a.nf;
-    const T2 zero{T(0), T(0)};
-    const T2 uA = q < a.nrow ? colA[(long)q * a.nm] : zero;
-    const T2 uAm = qm < a.nrow ? colA[(long)qm * a.nm] : zero;
-    const T2 uB = hasB && q < a.nrow ? colB[(long)q * a.nm] : zero;
-    const T2 uBm = hasB && qm < a.nrow ? colB[(long)qm * a.nm] : zero;
-    buf[k] = T2{T(0.5) * (uA.x + uB.x) + hs * (uAm.x - uBm.x), T(0.5) * (uA.y + uB.y) + hs * (uAm.y - uBm.y)};
-    if (km != k)
-      buf[km] = T2{T(0.5) * (uAm.x + uBm.x) + hs * (uA.x - uB.x), T(0.5) * (uAm.y + uBm.y) + hs * (uA.y - uB.y)};
+  // positions k and -k (the same for k = 0 and nf / 2) from the kept rows there, if any.  Every load of the
+  // thread is issued before the first LDS write (nf / 2 + 1 <= NMAX / 2 + 1 positions): the loads are
+  // unconditional, from clamped rows (row 0 and column A stand in), and zeroed afterwards.
+  {
+    constexpr int EK = (NMAX / 2 + NT) / NT;
+    const T2* colX = hasB ? colB : colA;
+    T2 uA[EK], uAm[EK], uB[EK], uBm[EK];
+#pragma unroll
+    for (int i = 0; i < EK; ++i) {
+      const int k = threadIdx.x + i * NT;
+      const int km = k == 0 ? 0 : a.nf - k;
+      int q = k - a.row_lo;
+      if (q >= a.nf) q -= a.nf;
+      int qm = km - a.row_lo;
+      if (qm >= a.nf) qm -= a.nf;
+      const bool in = k <= a.nf / 2;
+      const long oq = in && q < a.nrow ? (long)q * a.nm : 0, oqm = in && qm < a.nrow ? (long)qm * a.nm : 0;
+      uA[i] = colA[oq];
+      uAm[i] = colA[oqm];
+      uB[i] = colX[oq];
+      uBm[i] = colX[oqm];
+    }
+#pragma unroll
+    for (int i = 0; i < EK; ++i) {
+      const int k = threadIdx.x + i * NT;
+      const int km = k == 0 ? 0 : a.nf - k;
+      int q = k - a.row_lo;
+      if (q >= a.nf) q -= a.nf;
+      int qm = km - a.row_lo;
+      if (qm >= a.nf) qm -= a.nf;
+      const bool in = k <= a.nf / 2;
+      const T2 zero{T(0), T(0)};
+      if (!(in && q < a.nrow)) uA[i] = uB[i] = zero;
+      if (!(in && qm < a.nrow)) uAm[i] = uBm[i] = zero;
+      if (!hasB) uB[i] = uBm[i] = zero;
+    }
+#pragma unroll
+    for (int i = 0; i < EK; ++i) {
+      const int k = threadIdx.x + i * NT;
+      if (k <= a.nf / 2) {
+        const int km = k == 0 ? 0 : a.nf - k;
+        buf[k] = T2{T(0.5) * (uA[i].x + uB[i].x) + hs * (uAm[i].x - uBm[i].x),
+                    T(0.5) * (uA[i].y + uB[i].y) + hs * (uAm[i].y - uBm[i].y)};
+        if (km != k)
+          buf[km] = T2{T(0.5) * (uAm[i].x + uBm[i].x) + hs * (uA[i].x - uB[i].x),
+                       T(0.5) * (uAm[i].y + uBm[i].y) + hs * (uA[i].y - uB[i].y)};
+      }
+    }
   }
   __syncthreads();
   fft_lds<T, NT, NMAX, false, true, std::is_same<T, float>::value>(buf, a.fnf);
