@@ -120,13 +120,18 @@ __device__ __forceinline__ void dft_lds(c2_t<T>* buf, const DftLen<T>& L) {
   }
   if constexpr (METHOD != (int)DftMethod::Rader) return;
   // Rader (n prime): X[g^-p] = x0 + (a (*) b)_p with a_q = x[g^q], b_q = w^{g^-q}
+  // The perm_in and kern reads are unconditional, from positions clamped to n1 - 1, so that each thread's E
+  // reads of a table are in flight together (perm_out stays a guarded read: indices held in registers across
+  // the last step spill).
   const int n1 = n - 1;
   const T2 x0 = buf[0];
   T2 v[E];
+  {
+    int pin[E];
 #pragma unroll
-  for (int i = 0; i < E; ++i) {
-    int q = threadIdx.x + i * NT;
-    if (q < n1) v[i] = buf[L.perm_in[q]];
+    for (int i = 0; i < E; ++i) pin[i] = L.perm_in[min((int)threadIdx.x + i * NT, n1 - 1)];
+#pragma unroll
+    for (int i = 0; i < E; ++i) v[i] = buf[pin[i]];
   }
   __syncthreads();
 #pragma unroll
@@ -137,8 +142,14 @@ __device__ __forceinline__ void dft_lds(c2_t<T>* buf, const DftLen<T>& L) {
   __syncthreads();
   fft_lds<T, NT, kDftNMAX, true>(buf, L.fd);
   const T2 a0 = buf[0];
+#pragma unroll
+  for (int i = 0; i < E; ++i) v[i] = L.kern[min((int)threadIdx.x + i * NT, n1 - 1)];
   __syncthreads();
-  for (int q = threadIdx.x; q < n1; q += NT) buf[q] = cconj(cmul(buf[q], L.kern[q]));
+#pragma unroll
+  for (int i = 0; i < E; ++i) {
+    int q = threadIdx.x + i * NT;
+    if (q < n1) buf[q] = cconj(cmul(buf[q], v[i]));
+  }
   __syncthreads();
   fft_lds<T, NT, kDftNMAX, true>(buf, L.fd);
   const T inv = T(1) / T(n1);

@@ -65,12 +65,29 @@ __global__ __launch_bounds__(kRingNT) void k_ring(RingArgs a) {
     const int mlo = a.spin ? -L : 0;
     if (n >= 2 * L + 1) {
       for (int k = threadIdx.x; k < n; k += kRingNT) buf[k] = double2{0.0, 0.0};
+      // The thread's rows (at most kDftE: L - mlo < n <= kDftNMAX) are loaded unconditionally, from rows
+      // clamped to L, so that every load is in flight before the first LDS write.
+      double2 gv[kDftE];
+#pragma unroll
+      for (int i = 0; i < kDftE; ++i) {
+        const int m = min(mlo + (int)threadIdx.x + i * kRingNT, L);
+        gv[i] = g[(long)(a.spin ? m + L : m) * a.gstride];
+      }
       __syncthreads();
-      for (int m = mlo + (int)threadIdx.x; m <= L; m += kRingNT) {
-        const double2 v = ring_coef(g, m, L, a.spin, a.gstride, ph0);
-        int k = m >= 0 ? m : m + n;
-        buf[k] = cconj(v);
-        if (!a.spin && m > 0) buf[n - m] = v;
+#pragma unroll
+      for (int i = 0; i < kDftE; ++i) {
+        const int m = mlo + (int)threadIdx.x + i * kRingNT;
+        if (m <= L) {
+          double2 v = gv[i];
+          if (ph0 != 0.0) {
+            double s, c;
+            sincos(m * ph0, &s, &c);
+            v = cmul(v, double2{c, s});
+          }
+          const int k = m >= 0 ? m : m + n;
+          buf[k] = cconj(v);
+          if (!a.spin && m > 0) buf[n - m] = v;
+        }
       }
     } else {
       for (int k = threadIdx.x; k < n; k += kRingNT) {
