@@ -121,21 +121,22 @@ __global__ __launch_bounds__(NT) void k_theta2(ThetaArgs<T> a) {
   }
   __syncthreads();
   fft_lds<T, NT, NMAX, false, true, std::is_same<T, float>::value>(buf, a.fnf);
-  // rows of the kept band, both columns; spin 0 also fills the conjugate columns
-  const T hs = (T)(0.5 * sA);
-  for (int q = threadIdx.x; q < a.nrow; q += NT) {
+  // rows of the kept band, both columns; spin 0 also fills the conjugate columns.  A lane pair shares a row: the
+  // even lane writes column A, the odd lane column B, so the row's two values leave as one contiguous segment
+  // (half the rows per store instruction of one value per lane).
+  const int half = threadIdx.x & 1;
+  const T hs = (T)(half ? -0.5 * sA : 0.5 * sA);
+  const int m = half ? mB : mA;
+  for (int q = threadIdx.x >> 1; q < a.nrow; q += NT / 2) {
     int t = a.row_lo + q;
     t %= a.nf;
     if (t < 0) t += a.nf;
     const T2 s = buf[t], u = buf[t == 0 ? 0 : a.nf - t];
-    const T2 valA{T(0.5) * s.x + hs * u.x, -(T(0.5) * s.y + hs * u.y)};
+    const T2 val{T(0.5) * s.x + hs * u.x, -(T(0.5) * s.y + hs * u.y)};
     T2* orow = a.Tout + (long)q * a.nm;
-    orow[mA + a.lmax] = valA;
-    if (!a.spin && mA > 0) orow[a.lmax - mA] = T2{valA.x, -valA.y};
-    if (hasB) {
-      const T2 valB{T(0.5) * s.x - hs * u.x, -(T(0.5) * s.y - hs * u.y)};
-      orow[mB + a.lmax] = valB;
-      if (!a.spin) orow[a.lmax - mB] = T2{valB.x, -valB.y};
+    if (!half || hasB) {
+      orow[m + a.lmax] = val;
+      if (!a.spin && m > 0) orow[a.lmax - m] = T2{val.x, -val.y};
     }
   }
 }
