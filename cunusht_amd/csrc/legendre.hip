@@ -453,9 +453,10 @@ __global__ __launch_bounds__(64 * WPB) void k_synth(SynthArgs a) {
 // Operand layouts of v_mfma_f64_16x16x4_f64 on gfx950 (64 lanes):
 //   A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in register i / 4 of lane j + 16 (i % 4).
 
-// One thread per (block q of 4 l, chain, k, map b): the four columns of map b at l = l0 + 4q + k, one 32-byte
-// store (consecutive threads write consecutive 32 bytes).  Block row y serves m = y and m = lmax - y, whose
-// tables together have about lmax + 2 rows of l, so no block row is idle.
+// One thread per (block q of 4 l, k, map b): it reads map b's coefficients at l = l0 + 4q + k once and writes the
+// map's four columns of both chains, one 32-byte store per chain (consecutive threads write consecutive 32 bytes
+// of a chain).  Block row y serves m = y and m = lmax - y, whose tables together have about lmax + 2 rows of l, so
+// no block row is idle.
 __global__ void k_prep4(int lmax, int s, int nmaps, int ncomp, AlmPtrs4 alm,
                         const double* __restrict__ fl, const long* __restrict__ mofs, const long* __restrict__ mblk,
                         const double* __restrict__ cnorm, double* __restrict__ coef) {
@@ -468,39 +469,37 @@ __global__ void k_prep4(int lmax, int s, int nmaps, int ncomp, AlmPtrs4 alm,
     const int l0 = l0_of(m, s);
     if (l0 > lmax) continue;
     const int nl = lmax - l0 + 1;
-    const long nthr = (long)((nl + 3) >> 2) * nc * 16;
+    const long nthr = (long)((nl + 3) >> 2) * 16;
     const long mbase = (long)m * (2 * lmax + 1 - m) / 2;
     double* out = coef + mblk[m] * nc * 64;
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nthr; t += (long)gridDim.x * blockDim.x) {
       const int b = (int)(t & 3), k = (int)((t >> 2) & 3);
-      const long qc = t >> 4;
-      const int chain = (int)(qc % nc);
-      const long q = qc / nc;
+      const long q = t >> 4;
       const int jj = (int)(4 * q + k);
       const int l = l0 + jj;
-      double2 lo{0.0, 0.0}, hi{0.0, 0.0};  // columns c = 0, 1 and c = 2, 3
+      double2 w{0.0, 0.0}, wt{0.0, 0.0};  // spin 0: w = a_lm c_l
+      const double sl = ((l + m) & 1) ? -1.0 : 1.0;
       if (l <= lmax && b < nmaps) {
         double cl = cnorm[mofs[m] + jj];
         if (fl) cl *= fl[l];
         const double2* ab = b == 0 ? alm.p[0] : b == 1 ? alm.p[1] : b == 2 ? alm.p[2] : alm.p[3];
         const double2 ev = ab[mbase + l];
-        const double sl = ((l + m) & 1) ? -1.0 : 1.0;
         if (s == 0) {
-          lo = double2{cl * ev.x, cl * ev.y};
-          hi = double2{sl * lo.x, sl * lo.y};
+          w = double2{cl * ev.x, cl * ev.y};
         } else {
           const double2 bv = ncomp > 1 ? ab[nalm + mbase + l] : double2{0.0, 0.0};
-          const double2 w{sg * cl * (ev.x - bv.y), sg * cl * (ev.y + bv.x)};
-          const double2 wt{-cl * (ev.x + bv.y), -cl * (-ev.y + bv.x)};
-          // u chain: W, s_l Wt;  v chain: Wt, s_l W
-          const double2 z0 = chain == 0 ? w : wt, z1 = chain == 0 ? wt : w;
-          lo = z0;
-          hi = double2{sl * z1.x, sl * z1.y};
+          w = double2{sg * cl * (ev.x - bv.y), sg * cl * (ev.y + bv.x)};
+          wt = double2{-cl * (ev.x + bv.y), -cl * (-ev.y + bv.x)};
         }
       }
-      double2* o = reinterpret_cast<double2*>(out + (((q * nc + chain) * 4 + k) * 16 + b * 4));
-      o[0] = lo;
-      o[1] = hi;
+      // u chain (columns W, s_l Wt; spin 0: a, s_l a), then the v chain (Wt, s_l W) 64 doubles further
+      double2* o = reinterpret_cast<double2*>(out + ((q * nc * 4 + k) * 16 + b * 4));
+      o[0] = w;
+      o[1] = s == 0 ? double2{sl * w.x, sl * w.y} : double2{sl * wt.x, sl * wt.y};
+      if (s != 0) {
+        o[32] = wt;
+        o[33] = double2{sl * w.x, sl * w.y};
+      }
     }
   }
 }
@@ -1202,7 +1201,7 @@ static void legendre_synth_mfma(const LegendreTables& t, int nmaps, int ncomp, c
     StageScope sc(t.n_prep.c_str(), st);
     const int nc = t.spin ? 2 : 1;
     // block row y covers m = y and m = L - y; its x extent is sized for m = 0 (the longest table)
-    dim3 grid(ceil_div((L + 4) / 4 * nc * 16, 256), L / 2 + 1);
+    dim3 grid(ceil_div((L + 4) / 4 * 16, 256), L / 2 + 1);
     hipLaunchKernelGGL(k_prep4, grid, dim3(256), 0, st, L, t.spin, nmaps, ncomp, alm, fl,
                        t.mofs.as<long>(), t.mblk.as<long>(), t.cnorm.as<double>(), coef4);
     NUSHT_LAUNCH_CHECK();
