@@ -446,17 +446,18 @@ __global__ __launch_bounds__(64 * WPB) void k_synth(SynthArgs a) {
 //   spin s, u chain: Re W_b, Im W_b, Re s_l Wt_b, Im s_l Wt_b    (north +m, south -m)
 //           v chain: Re Wt_b, Im Wt_b, Re s_l W_b, Im s_l W_b    (north -m, south +m)
 //   spin 0, u chain: Re a_b, Im a_b, Re s_l a_b, Im s_l a_b      (north, south)
-// The table holds, per m and per block q of 4 l (l = l0 + 4q + k), 64 doubles per chain in the
-// B-operand order of the instruction: lane j + 16k holds column j of l0 + 4q + k.  Entries past
-// lmax are zero.
+// The table holds, per m and per block q of 4 l (l = l0 + 4q + k), the values once: spin s
+// [k][b][Re W, Im W, Re Wt, Im Wt] (64 doubles), spin 0 [k][b][Re a, Im a] (32).  Lane j + 16k of the
+// B operand (column j = 4b + c at l0 + 4q + k) reads entry (k, b, c) for the u chain and (k, b, c ^ 2) for
+// the v chain (spin s), (k, b, c & 1) for spin 0, and negates it when c >= 2 and l + m is odd; blocks start
+// at l0 + 4q, so that sign is fixed per lane.  Entries past lmax are zero.
 
 // Operand layouts of v_mfma_f64_16x16x4_f64 on gfx950 (64 lanes):
 //   A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in register i / 4 of lane j + 16 (i % 4).
 
-// One thread per (block q of 4 l, k, map b): it reads map b's coefficients at l = l0 + 4q + k once and writes the
-// map's four columns of both chains, one 32-byte store per chain (consecutive threads write consecutive 32 bytes
-// of a chain).  Block row y serves m = y and m = lmax - y, whose tables together have about lmax + 2 rows of l, so
-// no block row is idle.
+// One thread per (block q of 4 l, k, map b): map b's entry at l = l0 + 4q + k, one 32-byte store (16 for spin 0;
+// consecutive threads write consecutive entries).  Block row y serves m = y and m = lmax - y, whose tables together
+// have about lmax + 2 rows of l, so no block row is idle.
 __global__ void k_prep4(int lmax, int s, int nmaps, int ncomp, AlmPtrs4 alm,
                         const double* __restrict__ fl, const long* __restrict__ mofs, const long* __restrict__ mblk,
                         const double* __restrict__ cnorm, double* __restrict__ coef) {
@@ -471,14 +472,13 @@ __global__ void k_prep4(int lmax, int s, int nmaps, int ncomp, AlmPtrs4 alm,
     const int nl = lmax - l0 + 1;
     const long nthr = (long)((nl + 3) >> 2) * 16;
     const long mbase = (long)m * (2 * lmax + 1 - m) / 2;
-    double* out = coef + mblk[m] * nc * 64;
+    double* out = coef + mblk[m] * nc * 32;
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nthr; t += (long)gridDim.x * blockDim.x) {
       const int b = (int)(t & 3), k = (int)((t >> 2) & 3);
       const long q = t >> 4;
       const int jj = (int)(4 * q + k);
       const int l = l0 + jj;
       double2 w{0.0, 0.0}, wt{0.0, 0.0};  // spin 0: w = a_lm c_l
-      const double sl = ((l + m) & 1) ? -1.0 : 1.0;
       if (l <= lmax && b < nmaps) {
         double cl = cnorm[mofs[m] + jj];
         if (fl) cl *= fl[l];
@@ -492,14 +492,9 @@ __global__ void k_prep4(int lmax, int s, int nmaps, int ncomp, AlmPtrs4 alm,
           wt = double2{-cl * (ev.x + bv.y), -cl * (-ev.y + bv.x)};
         }
       }
-      // u chain (columns W, s_l Wt; spin 0: a, s_l a), then the v chain (Wt, s_l W) 64 doubles further
-      double2* o = reinterpret_cast<double2*>(out + ((q * nc * 4 + k) * 16 + b * 4));
+      double2* o = reinterpret_cast<double2*>(out + q * nc * 32 + (k * 4 + b) * nc * 2);
       o[0] = w;
-      o[1] = s == 0 ? double2{sl * w.x, sl * w.y} : double2{sl * wt.x, sl * wt.y};
-      if (s != 0) {
-        o[32] = wt;
-        o[33] = double2{sl * w.x, sl * w.y};
-      }
+      if (s != 0) o[1] = wt;
     }
   }
 }
@@ -555,7 +550,16 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     if (SP) { mu_v = sm.y; pv = sp.y; kv = sk.y; }
   }
   const double2* __restrict__ rec = a.rec + a.mofs[m];
-  const double* __restrict__ cb = coef4 + mblk[m] * NC * 64;
+  const double* __restrict__ cb = coef4 + mblk[m] * NC * 32;
+  // this lane's B entries in a block (u chain, v chain) and their sign bit: column c = lane & 3 >= 2 at odd l + m,
+  // l = l0 + 4q + (lane >> 4)
+  const int c4 = lane & 3;
+  const int eu = SP ? lane : (lane >> 4) * 8 + ((lane & 15) >> 2) * 2 + (c4 & 1);
+  const int ev = lane ^ 2;
+  const unsigned sgnb = (c4 >= 2 && ((l0 + (lane >> 4) + m) & 1)) ? 0x80000000u : 0u;
+  auto sgn = [&](double v) {
+    return __hiloint2double(__double2hiint(v) ^ (int)sgnb, __double2loint(v));
+  };
 
   v4d_t Du[4], Dv[4];
 #pragma unroll
@@ -571,9 +575,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     const int jl = lc - l0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) b.rc[i] = rec[jl + i];
-    const double* blkp = cb + (long)(jl >> 2) * NC * 64;
-    b.bu = blkp[lane];
-    if (SP) b.bv = blkp[64 + lane];
+    const double* blkp = cb + (long)(jl >> 2) * NC * 32;
+    b.bu = blkp[eu];
+    if (SP) b.bv = blkp[ev];
   };
   auto chunk = [&](int lc, const Buf& bf, auto slow_tag, auto full_tag) {
     constexpr bool SLOW = decltype(slow_tag)::value;
@@ -598,12 +602,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
       }
     }
     transpose_rows(U);
+    const double bu = sgn(bf.bu);
 #pragma unroll
-    for (int g = 0; g < 4; ++g) Du[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(U[g], bf.bu, Du[g], 0, 0, 0);
+    for (int g = 0; g < 4; ++g) Du[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(U[g], bu, Du[g], 0, 0, 0);
     if (SP) {
       transpose_rows(V);
+      const double bv = sgn(bf.bv);
 #pragma unroll
-      for (int g = 0; g < 4; ++g) Dv[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(V[g], bf.bv, Dv[g], 0, 0, 0);
+      for (int g = 0; g < 4; ++g) Dv[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(V[g], bv, Dv[g], 0, 0, 0);
     }
   };
   auto any_under = [&]() { return __any((ku < 0.0) || (kv < 0.0)); };
@@ -1106,8 +1112,8 @@ size_t legendre_coef_bytes(const LegendreTables& t, int nmaps) {
   const size_t vec = sizeof(double2) * (size_t)(std::max<long>(t.nlm, 1) + 2 * kSynCH) * padded_maps(nmaps) *
                      (t.spin ? 2 : 1);
   if (nmaps < 3) return vec;
-  // matrix-core layout: 64 doubles per chain per block of 4 l, 2 blocks of padding (prefetch)
-  const size_t mf = sizeof(double) * 64 * (t.spin ? 2 : 1) * (size_t)(t.nblk + 2);
+  // matrix-core layout: 32 doubles per chain per block of 4 l, 2 blocks of padding (prefetch)
+  const size_t mf = sizeof(double) * 32 * (t.spin ? 2 : 1) * (size_t)(t.nblk + 2);
   return std::max(vec, mf);
 }
 
