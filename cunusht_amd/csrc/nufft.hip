@@ -382,8 +382,17 @@ __device__ __forceinline__ void interp_mark_bad(const InterpArgs<T>& a) {
   }
 }
 
+// Points per tile that the interpolation orders by bank class in LDS (tiles with more keep the binned order).
+constexpr int kInterpSortCap = 1024;
+
 // One workgroup per fine-grid tile: the tile plus its (W-1)-cell halo is staged in LDS once and
 // every point binned to the tile is interpolated from it (fully unrolled W x W footprint).
+// A 16-byte LDS read serves 16 lanes per pass, and lane k's reads fall in bank group (base_k + offset) mod 16,
+// base_k = lr * tw + lc the footprint's first cell, offset the same for all lanes; consecutive points of a ring
+// are about 1.25 cells apart, so 16 of them span about 20 columns and collide.  The workgroup therefore counts
+// its points per class base mod 16 (LDS atomics), lists them class by class, and gives lane position k the
+// list entry (k mod 16) S + k / 16, S = ceil(n / 16): the 16 lanes of a pass take entries S apart, from
+// (nearly always) different classes.
 template <typename T, int W>
 __global__ __launch_bounds__(256) void k_interp_w(InterpArgs<T> a) {
   using T2 = c2_t<T>;
@@ -397,6 +406,11 @@ __global__ __launch_bounds__(256) void k_interp_w(InterpArgs<T> a) {
   const int s0 = a.tile_start[tb], s1 = a.tile_start[tb + 1];
   if (s0 == s1) return;
   const int tw = a.tc + W, th = a.tr + W;
+  int* ccnt = reinterpret_cast<int*>(smem_raw + sizeof(T2) * (size_t)tw * th);  // [16] class counts, then starts
+  int* order = ccnt + 16;                                                          // [kInterpSortCap]
+  const int n = s1 - s0;
+  const bool sorted = n <= kInterpSortCap;
+  if (sorted && threadIdx.x < 16) ccnt[threadIdx.x] = 0;
   const int tile_r = tb / a.ntc, tile_c = tb - (tb / a.ntc) * a.ntc;
   const int r0 = tile_r * a.tr, c0 = tile_c * a.tc;
   for (int i = threadIdx.x; i < tw * th; i += blockDim.x) {
@@ -409,15 +423,62 @@ __global__ __launch_bounds__(256) void k_interp_w(InterpArgs<T> a) {
   __syncthreads();
   const double scale = a.nf / (2.0 * M_PI);
   const double hw = 0.5 * W;
-  for (int p = s0 + threadIdx.x; p < s1; p += blockDim.x) {
-    const double2 tp = a.pts[p];
-    const double ut = tp.x * scale;
-    double up = tp.y * scale;
+  // first cell of a point's footprint relative to the tile
+  auto first_cell = [&](double2 tp, double& ut, double& up, int& i0, int& j0u, int& lr, int& lc) {
+    ut = tp.x * scale;
+    up = tp.y * scale;
     up -= a.nf * floor(up / a.nf);
-    const int i0 = (int)ceil(ut - hw);
-    const int j0u = (int)ceil(up - hw);
+    i0 = (int)ceil(ut - hw);
+    j0u = (int)ceil(up - hw);
     const int j0 = j0u < 0 ? j0u + a.nf : j0u;
-    const int lr = i0 - a.row_lo - r0, lc = j0 - c0;
+    lr = i0 - a.row_lo - r0;
+    lc = j0 - c0;
+  };
+  if (sorted) {
+    constexpr int EI = kInterpSortCap / 256;
+    int cls[EI], rank[EI];
+#pragma unroll
+    for (int e = 0; e < EI; ++e) {
+      const int i = threadIdx.x + e * 256;
+      cls[e] = rank[e] = 0;
+      if (i < n) {
+        double ut, up;
+        int i0, j0u, lr, lc;
+        first_cell(a.pts[s0 + i], ut, up, i0, j0u, lr, lc);
+        cls[e] = (lr * tw + lc) & 15;
+        rank[e] = atomicAdd(&ccnt[cls[e]], 1);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int acc = 0;
+      for (int c = 0; c < 16; ++c) {
+        const int v = ccnt[c];
+        ccnt[c] = acc;
+        acc += v;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < EI; ++e) {
+      const int i = threadIdx.x + e * 256;
+      if (i < n) order[ccnt[cls[e]] + rank[e]] = i;
+    }
+    __syncthreads();
+  }
+  const int S = (n + 15) >> 4;
+  const int nk = sorted ? 16 * S : n;
+  for (int k = threadIdx.x; k < nk; k += blockDim.x) {
+    int pi = k;
+    if (sorted) {
+      const int q = (k & 15) * S + (k >> 4);
+      if (q >= n) continue;
+      pi = order[q];
+    }
+    const int p = s0 + pi;
+    double ut, up;
+    int i0, j0u, lr, lc;
+    first_cell(a.pts[p], ut, up, i0, j0u, lr, lc);
     const int o = a.idx[p];
     const bool ok = lr >= 0 && lr < a.tr && lc >= 0 && lc < a.tc;  // always true for binned points
     T kt[W], kp[W];
@@ -1508,7 +1569,8 @@ void NufftPlan::type2_impl(const double2* G, long sm, const double* gamma, doubl
                    kpoly_.as<T>(), p.poly_deg, npts_, pts_.as<double2>(), idx_.as<int>(), tile_start_.as<int>(),
                    grid_.as<T2>(), gamma, ore, oim};
   StageScope sc("nufft_interp", st);
-  const size_t tlds = sizeof(T2) * (size_t)(tile_r_ + p.w) * (tile_c_ + p.w);
+  // tile, then (templated widths) the class counts and the point order
+  const size_t tlds = sizeof(T2) * (size_t)(tile_r_ + p.w) * (tile_c_ + p.w) + sizeof(int) * (16 + kInterpSortCap);
   launch_interp<T, 2>(p.w, dim3(ntiles + 1), tlds, st, ia);  // + 1: the workgroup for out-of-range points
   NUSHT_LAUNCH_CHECK();
 }
