@@ -309,6 +309,33 @@ def test_nusht2d2_vs_direct(spin, eps):
 
 
 @pytest.mark.parametrize("spin", [0, 2])
+@pytest.mark.parametrize("eps", [1e-10, 1e-7])
+def test_nusht2d2_dense_tiles(spin, eps):
+    """Tiles above the interpolation's per-tile ordering cap (1024 points, kept in binned order) beside ordinary
+    tiles: 5000 points in a 0.05 rad patch (less than one fine cell at lmax 30) plus 1500 uniform ones, against
+    direct summation."""
+    import cunusht_amd
+    rng = np.random.default_rng(97 + spin)
+    L = 30
+    nd, nu = 5000, 1500
+    theta = np.concatenate([1.0 + 0.05 * rng.uniform(0, 1, nd), np.arccos(rng.uniform(-1, 1, nu))])
+    phi = np.concatenate([2.0 + 0.05 * rng.uniform(0, 1, nd), rng.uniform(0, 2 * np.pi, nu)])
+    perm = rng.permutation(nd + nu)
+    theta, phi = theta[perm], phi[perm]
+    loc = np.stack([theta, phi], 1)
+    nc = 1 if spin == 0 else 2
+    a = almutil.random_alm(L, spin, nc, rng)
+    ain = a[0] if spin == 0 else a
+    T = cunusht_amd.get_transformer("GPU")(("gl", {"lmax": L}), nuFFTtype=2, epsilon=eps, spin=spin)
+    got = T.nusht2d2(L, L, _t(ain), _t(loc), spin=spin).cpu().numpy()
+    ref = osht.synth_direct(ain, L, spin, theta, phi)
+    if spin:
+        ref = np.array([ref.real, ref.imag])
+    err = _rel2(got, ref)
+    assert err < 10 * max(eps, 1e-13), err
+
+
+@pytest.mark.parametrize("spin", [0, 2])
 @pytest.mark.parametrize("eps", [1e-10, 1e-5])
 def test_nusht2d1_adjointness(spin, eps):
     """Dot-product test <x, A y> = <A^H x, y> (test/test_adjointOperator.py:247-277, rtol 1e-9)."""
