@@ -574,7 +574,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   auto load = [&](int lc, Buf& b) {
     const int jl = lc - l0;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) b.rc[i] = rec[jl + i];
+    for (int i = 0; i < 4; ++i) b.rc[i] = rec[jl + 1 + i];  // the steps from l + 1 (state one step ahead)
     const double* blkp = cb + (long)(jl >> 2) * NC * 32;
     b.bu = blkp[eu];
     if (SP) b.bv = blkp[ev];
@@ -586,7 +586,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       if (!FULL && lc + i > L) { U[i] = 0.0; V[i] = 0.0; continue; }  // zero weight past lmax
-      double wu = mu_u, wv = mu_v;
+      double wu = pu, wv = pv;  // the value at lc + i
       if (SLOW) { wu = ku == 0.0 ? wu : 0.0; wv = kv == 0.0 ? wv : 0.0; }
       U[i] = wu; V[i] = wv;
       const double t = bf.rc[i].x * x;
@@ -615,6 +615,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
   auto any_under = [&]() { return __any((ku < 0.0) || (kv < 0.0)); };
   using T_ = std::true_type;
   using F_ = std::false_type;
+  // The l-loop runs one recursion step ahead: at the first l of a chunk it holds (mu, p) = (mu_{l+1}, mu_l), and
+  // the chunk's four A values are p before each step, so none of them is still part of the recursion state when
+  // the transposes overwrite them in place (no register copies).  The step to l + 1 is made here; a step past
+  // lmax reads a padded or neighbouring table entry and its result is never used.
+  {
+    const double2 rc0 = rec[lstart - l0];
+    const double t = rc0.x * x;
+    const double nu = fma(t + rc0.y, mu_u, -pu);
+    pu = mu_u; mu_u = nu;
+    if (SP) {
+      const double nv = fma(t - rc0.y, mu_v, -pv);
+      pv = mu_v; mu_v = nv;
+    }
+    if (ku < 0.0 && fabs(mu_u) > kThresh) { mu_u *= kSmall; pu *= kSmall; ku += 1.0; }
+    if (SP && kv < 0.0 && fabs(mu_v) > kThresh) { mu_v *= kSmall; pv *= kSmall; kv += 1.0; }
+  }
   int lc = lstart;
   Buf cur, nxt;
   load(lc, cur);
