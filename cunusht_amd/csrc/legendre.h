@@ -63,15 +63,17 @@ struct LegendreTables {
 // (negative m -> lmax - |m|); element (row, ring) at G + row * g_stride_m + ring * g_stride_r (+ map * g_stride_map).
 // fl (optional, device double[lmax+1]) multiplies E by fl[l] before synthesis.
 size_t legendre_coef_bytes(const LegendreTables& t, int nmaps);
-// Maps per launch of the matrix-core synthesis path (used for batches of at least 3 maps).
+// Maps per launch of the matrix-core synthesis path (used for batches of at least 3 maps); batches of at least
+// kLegendreMfma8Min maps take up to 8 per launch (two 16-column groups sharing one recursion).
 constexpr int kLegendreMfmaMaps = 4;
+constexpr int kLegendreMfma8Min = 7;
 void legendre_synth(const LegendreTables& t, int nmaps, int ncomp, const double2* alm, long alm_stride_map,
                     const double* fl, double2* coef_ws, double2* G, long g_stride_m, long g_stride_r,
                     long g_stride_map, hipStream_t st);
 
-// Per-map alm of a matrix-core launch (up to kLegendreMfmaMaps maps at their own addresses).
-struct AlmPtrs4 {
-  const double2* p[4];
+// Per-map alm of a matrix-core launch (up to 8 maps at their own addresses).
+struct AlmPtrs8 {
+  const double2* p[8];
 };
 // The same synthesis for nmaps maps whose alm lie at separate addresses (one pointer per map, host array);
 // G as in legendre_synth.

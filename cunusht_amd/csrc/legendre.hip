@@ -446,11 +446,12 @@ __global__ __launch_bounds__(64 * WPB) void k_synth(SynthArgs a) {
 //   spin s, u chain: Re W_b, Im W_b, Re s_l Wt_b, Im s_l Wt_b    (north +m, south -m)
 //           v chain: Re Wt_b, Im Wt_b, Re s_l W_b, Im s_l W_b    (north -m, south +m)
 //   spin 0, u chain: Re a_b, Im a_b, Re s_l a_b, Im s_l a_b      (north, south)
+// A launch takes 4 NS maps (NS = 1 or 2 groups of 16 columns; the recursion and the A tiles serve both groups).
 // The table holds, per m and per block q of 4 l (l = l0 + 4q + k), the values once: spin s
-// [k][b][Re W, Im W, Re Wt, Im Wt] (64 doubles), spin 0 [k][b][Re a, Im a] (32).  Lane j + 16k of the
-// B operand (column j = 4b + c at l0 + 4q + k) reads entry (k, b, c) for the u chain and (k, b, c ^ 2) for
-// the v chain (spin s), (k, b, c & 1) for spin 0, and negates it when c >= 2 and l + m is odd; blocks start
-// at l0 + 4q, so that sign is fixed per lane.  Entries past lmax are zero.
+// [k][b][Re W, Im W, Re Wt, Im Wt] (64 NS doubles), spin 0 [k][b][Re a, Im a] (32 NS), b < 4 NS.  Lane j + 16k of
+// group g's B operand (column j = 4b' + c of map b = 4g + b' at l0 + 4q + k) reads entry (k, b, c) for the u
+// chain and (k, b, c ^ 2) for the v chain (spin s), (k, b, c & 1) for spin 0, and negates it when c >= 2 and
+// l + m is odd; blocks start at l0 + 4q, so that sign is fixed per lane.  Entries past lmax are zero.
 
 // Operand layouts of v_mfma_f64_16x16x4_f64 on gfx950 (64 lanes):
 //   A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in register i / 4 of lane j + 16 (i % 4).
@@ -458,7 +459,8 @@ __global__ __launch_bounds__(64 * WPB) void k_synth(SynthArgs a) {
 // One thread per (block q of 4 l, k, map b): map b's entry at l = l0 + 4q + k, one 32-byte store (16 for spin 0;
 // consecutive threads write consecutive entries).  Block row y serves m = y and m = lmax - y, whose tables together
 // have about lmax + 2 rows of l, so no block row is idle.
-__global__ void k_prep4(int lmax, int s, int nmaps, int ncomp, AlmPtrs4 alm,
+template <int NS>
+__global__ void k_prep4(int lmax, int s, int nmaps, int ncomp, AlmPtrs8 alm,
                         const double* __restrict__ fl, const long* __restrict__ mofs, const long* __restrict__ mblk,
                         const double* __restrict__ cnorm, double* __restrict__ coef) {
   const int nc = s ? 2 : 1;
@@ -470,19 +472,22 @@ __global__ void k_prep4(int lmax, int s, int nmaps, int ncomp, AlmPtrs4 alm,
     const int l0 = l0_of(m, s);
     if (l0 > lmax) continue;
     const int nl = lmax - l0 + 1;
-    const long nthr = (long)((nl + 3) >> 2) * 16;
+    constexpr int NM = 4 * NS, LNM = NS == 1 ? 2 : 3;  // maps per launch and log2
+    const long nthr = (long)((nl + 3) >> 2) * 4 * NM;
     const long mbase = (long)m * (2 * lmax + 1 - m) / 2;
-    double* out = coef + mblk[m] * nc * 32;
+    double* out = coef + mblk[m] * nc * 32 * NS;
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < nthr; t += (long)gridDim.x * blockDim.x) {
-      const int b = (int)(t & 3), k = (int)((t >> 2) & 3);
-      const long q = t >> 4;
+      const int b = (int)(t & (NM - 1)), k = (int)((t >> LNM) & 3);
+      const long q = t >> (LNM + 2);
       const int jj = (int)(4 * q + k);
       const int l = l0 + jj;
       double2 w{0.0, 0.0}, wt{0.0, 0.0};  // spin 0: w = a_lm c_l
       if (l <= lmax && b < nmaps) {
         double cl = cnorm[mofs[m] + jj];
         if (fl) cl *= fl[l];
-        const double2* ab = b == 0 ? alm.p[0] : b == 1 ? alm.p[1] : b == 2 ? alm.p[2] : alm.p[3];
+        const double2* ab = alm.p[0];  // selects, not an indexed read of the argument array
+#pragma unroll
+        for (int u = 1; u < NM; ++u) ab = b == u ? alm.p[u] : ab;
         const double2 ev = ab[mbase + l];
         if (s == 0) {
           w = double2{cl * ev.x, cl * ev.y};
@@ -492,7 +497,7 @@ __global__ void k_prep4(int lmax, int s, int nmaps, int ncomp, AlmPtrs4 alm,
           wt = double2{-cl * (ev.x + bv.y), -cl * (-ev.y + bv.x)};
         }
       }
-      double2* o = reinterpret_cast<double2*>(out + q * nc * 32 + (k * 4 + b) * nc * 2);
+      double2* o = reinterpret_cast<double2*>(out + q * nc * 32 * NS + (k * NM + b) * nc * 2);
       o[0] = w;
       if (s != 0) o[1] = wt;
     }
@@ -524,9 +529,9 @@ __device__ __forceinline__ void transpose_rows(double (&u)[4]) {
   for (int k = 0; k < 4; ++k) u[k] = __hiloint2double((int)hi[k], (int)lo[k]);
 }
 
-template <bool SP>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_synth_mfma(SynthArgs a, const double* __restrict__ coef4,
-                                                   const long* __restrict__ mblk) {
+template <bool SP, int NS>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NS == 1 ? 3 : 2))) void k_synth_mfma(
+    SynthArgs a, const double* __restrict__ coef4, const long* __restrict__ mblk) {
   constexpr int NC = SP ? 2 : 1;
   // same block -> (m, slot group) mapping as k_synth with one wave per block
   const int blk = blockIdx.x;
@@ -550,38 +555,45 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     if (SP) { mu_v = sm.y; pv = sp.y; kv = sk.y; }
   }
   const double2* __restrict__ rec = a.rec + a.mofs[m];
-  const double* __restrict__ cb = coef4 + mblk[m] * NC * 32;
-  // this lane's B entries in a block (u chain, v chain) and their sign bit: column c = lane & 3 >= 2 at odd l + m,
-  // l = l0 + 4q + (lane >> 4)
+  const double* __restrict__ cb = coef4 + mblk[m] * NC * 32 * NS;
+  // this lane's B entries in a block for group 0 (u chain, v chain; group g adds 16 g, 8 g for spin 0) and their
+  // sign bit: column c = lane & 3 >= 2 at odd l + m, l = l0 + 4q + (lane >> 4)
   const int c4 = lane & 3;
-  const int eu = SP ? lane : (lane >> 4) * 8 + ((lane & 15) >> 2) * 2 + (c4 & 1);
-  const int ev = lane ^ 2;
+  const int eu = SP ? (lane >> 4) * 16 * NS + (lane & 15) : (lane >> 4) * 8 * NS + ((lane & 15) >> 2) * 2 + (c4 & 1);
+  const int ev = eu ^ 2;
   const unsigned sgnb = (c4 >= 2 && ((l0 + (lane >> 4) + m) & 1)) ? 0x80000000u : 0u;
   auto sgn = [&](double v) {
     return __hiloint2double(__double2hiint(v) ^ (int)sgnb, __double2loint(v));
   };
 
-  v4d_t Du[4], Dv[4];
+  v4d_t Du[NS][4], Dv[NS][4];
 #pragma unroll
-  for (int g = 0; g < 4; ++g) {
-    Du[g] = v4d_t{0.0, 0.0, 0.0, 0.0};
-    Dv[g] = v4d_t{0.0, 0.0, 0.0, 0.0};
-  }
+  for (int h = 0; h < NS; ++h)
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      Du[h][g] = v4d_t{0.0, 0.0, 0.0, 0.0};
+      Dv[h][g] = v4d_t{0.0, 0.0, 0.0, 0.0};
+    }
   struct Buf {
     double2 rc[4];
-    double bu, bv;
+    double bu[NS], bv[NS];
   };
   auto load = [&](int lc, Buf& b) {
     const int jl = lc - l0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) b.rc[i] = rec[jl + 1 + i];  // the steps from l + 1 (state one step ahead)
-    const double* blkp = cb + (long)(jl >> 2) * NC * 32;
-    b.bu = blkp[eu];
-    if (SP) b.bv = blkp[ev];
+    const double* blkp = cb + (long)(jl >> 2) * NC * 32 * NS;
+#pragma unroll
+    for (int h = 0; h < NS; ++h) {
+      b.bu[h] = blkp[eu + h * (SP ? 16 : 8)];
+      if (SP) b.bv[h] = blkp[ev + h * 16];
+    }
   };
-  auto chunk = [&](int lc, const Buf& bf, auto slow_tag, auto full_tag) {
+  // GRP: the map groups a chunk accumulates (-1 all, else that group alone)
+  auto chunk = [&](int lc, const Buf& bf, auto slow_tag, auto full_tag, auto grp_tag) {
     constexpr bool SLOW = decltype(slow_tag)::value;
     constexpr bool FULL = decltype(full_tag)::value;
+    constexpr int GRP = decltype(grp_tag)::value;
     double U[4], V[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -602,14 +614,22 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
       }
     }
     transpose_rows(U);
-    const double bu = sgn(bf.bu);
 #pragma unroll
-    for (int g = 0; g < 4; ++g) Du[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(U[g], bu, Du[g], 0, 0, 0);
+    for (int h = 0; h < NS; ++h) {
+      if (GRP >= 0 && h != GRP) continue;
+      const double bu = sgn(bf.bu[h]);
+#pragma unroll
+      for (int g = 0; g < 4; ++g) Du[h][g] = __builtin_amdgcn_mfma_f64_16x16x4f64(U[g], bu, Du[h][g], 0, 0, 0);
+    }
     if (SP) {
       transpose_rows(V);
-      const double bv = sgn(bf.bv);
 #pragma unroll
-      for (int g = 0; g < 4; ++g) Dv[g] = __builtin_amdgcn_mfma_f64_16x16x4f64(V[g], bv, Dv[g], 0, 0, 0);
+      for (int h = 0; h < NS; ++h) {
+        if (GRP >= 0 && h != GRP) continue;
+        const double bv = sgn(bf.bv[h]);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) Dv[h][g] = __builtin_amdgcn_mfma_f64_16x16x4f64(V[g], bv, Dv[h][g], 0, 0, 0);
+      }
     }
   };
   auto any_under = [&]() { return __any((ku < 0.0) || (kv < 0.0)); };
@@ -631,62 +651,86 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     if (ku < 0.0 && fabs(mu_u) > kThresh) { mu_u *= kSmall; pu *= kSmall; ku += 1.0; }
     if (SP && kv < 0.0 && fabs(mu_v) > kThresh) { mu_v *= kSmall; pv *= kSmall; kv += 1.0; }
   }
+  using GA_ = std::integral_constant<int, -1>;
   int lc = lstart;
   Buf cur, nxt;
   load(lc, cur);
-  // slow phase (some lane below 2^-300), fast phase, partial last chunk; the next chunk's
-  // coefficients are loaded before the current chunk is computed
-  while (lc <= L && any_under()) {
-    load(lc + 4, nxt);
-    if (lc + 3 <= L) chunk(lc, cur, T_{}, T_{});
-    else chunk(lc, cur, T_{}, F_{});
-    cur = nxt;
-    lc += 4;
+  // slow phase (some lane below 2^-300); the next chunk's coefficients are loaded before the current chunk is
+  // computed.  With two map groups it runs once per group, the recursion restarting from its saved state for the
+  // second: a group's accumulators are then either still zero or untouched in the loop, and the loop keeps only
+  // the other group's in registers (both together with the rescaling state exceed the two-wave register budget).
+  // The second pass runs the same number of chunks, since the phase's end depends on the recursion alone.
+  auto slow_phase = [&](auto grp_tag, int lend) {
+    while (lend < 0 ? (lc <= L && any_under()) : lc < lend) {
+      load(lc + 4, nxt);
+      if (lc + 3 <= L) chunk(lc, cur, T_{}, T_{}, grp_tag);
+      else chunk(lc, cur, T_{}, F_{}, grp_tag);
+      cur = nxt;
+      lc += 4;
+    }
+  };
+  if constexpr (NS == 1) {
+    slow_phase(GA_{}, -1);
+  } else {
+    const double s_mu_u = mu_u, s_pu = pu, s_mu_v = mu_v, s_pv = pv, s_ku = ku, s_kv = kv;
+    slow_phase(std::integral_constant<int, 0>{}, -1);
+    const int lend = lc;
+    if (lend > lstart) {
+      mu_u = s_mu_u; pu = s_pu; mu_v = s_mu_v; pv = s_pv; ku = s_ku; kv = s_kv;
+      lc = lstart;
+      load(lc, cur);
+      slow_phase(std::integral_constant<int, 1>{}, lend);
+    }
   }
   // two chunks per iteration, so the two buffers alternate without copies (loads reach at most one 4-l block
   // past lmax, as the single-chunk step does)
   while (lc + 7 <= L) {
     load(lc + 4, nxt);
-    chunk(lc, cur, F_{}, T_{});
+    chunk(lc, cur, F_{}, T_{}, GA_{});
     load(lc + 8, cur);
-    chunk(lc + 4, nxt, F_{}, T_{});
+    chunk(lc + 4, nxt, F_{}, T_{}, GA_{});
     lc += 8;
   }
   if (lc + 3 <= L) {
     load(lc + 4, nxt);
-    chunk(lc, cur, F_{}, T_{});
+    chunk(lc, cur, F_{}, T_{}, GA_{});
     cur = nxt;
     lc += 4;
   }
-  if (lc <= L) chunk(lc, cur, F_{}, F_{});
+  if (lc <= L) chunk(lc, cur, F_{}, F_{}, GA_{});
 
-  // epilogue: lane = jc + 16 qq holds column jc for ring pairs 16 g + 4 r + qq (register r)
+  // epilogue: lane = jc + 16 qq holds column jc of group h (map 4 h + jc / 4) for ring pairs 16 g + 4 r + qq
+  // (register r)
   const int jc = lane & 15, qq = lane >> 4;
-  const int b = jc >> 2, c = jc & 3;
-  if (b >= a.nmaps) return;
-  double* Gd = reinterpret_cast<double*>(a.G + b * a.smap);
+  const int c = jc & 3;
   const int part = c & 1;
 #pragma unroll
-  for (int g = 0; g < 4; ++g) {
+  for (int h = 0; h < NS; ++h) {
+    const int b = 4 * h + (jc >> 2);
+    if (b >= a.nmaps) continue;
+    double* Gd = reinterpret_cast<double*>(a.G + b * a.smap);
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int sl = sb * 64 + 16 * g + 4 * r + qq;
-      if (sl >= a.nslot) continue;
-      const int rn = a.slot_north[sl];
-      const int rs = a.slot_south[sl];
-      const double du = Du[g][r];
-      if (!SP) {
-        if (c < 2) Gd[((long)m * a.sm + (long)rn * a.sr) * 2 + part] = du;
-        else if (rs >= 0) Gd[((long)m * a.sm + (long)rs * a.sr) * 2 + part] = du;
-      } else {
-        const double dv = Dv[g][r];
-        const long rowp = (long)(L + m) * a.sm, rowm = (long)(L - m) * a.sm;
-        if (c < 2) {
-          Gd[(rowp + (long)rn * a.sr) * 2 + part] = du;                        // north +m
-          if (m > 0) Gd[(rowm + (long)rn * a.sr) * 2 + part] = dv;            // north -m
-        } else if (rs >= 0) {
-          Gd[(rowp + (long)rs * a.sr) * 2 + part] = dv;                        // south +m
-          if (m > 0) Gd[(rowm + (long)rs * a.sr) * 2 + part] = du;            // south -m
+    for (int g = 0; g < 4; ++g) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int sl = sb * 64 + 16 * g + 4 * r + qq;
+        if (sl >= a.nslot) continue;
+        const int rn = a.slot_north[sl];
+        const int rs = a.slot_south[sl];
+        const double du = Du[h][g][r];
+        if (!SP) {
+          if (c < 2) Gd[((long)m * a.sm + (long)rn * a.sr) * 2 + part] = du;
+          else if (rs >= 0) Gd[((long)m * a.sm + (long)rs * a.sr) * 2 + part] = du;
+        } else {
+          const double dv = Dv[h][g][r];
+          const long rowp = (long)(L + m) * a.sm, rowm = (long)(L - m) * a.sm;
+          if (c < 2) {
+            Gd[(rowp + (long)rn * a.sr) * 2 + part] = du;                        // north +m
+            if (m > 0) Gd[(rowm + (long)rn * a.sr) * 2 + part] = dv;            // north -m
+          } else if (rs >= 0) {
+            Gd[(rowp + (long)rs * a.sr) * 2 + part] = dv;                        // south +m
+            if (m > 0) Gd[(rowm + (long)rs * a.sr) * 2 + part] = du;            // south -m
+          }
         }
       }
     }
@@ -1128,8 +1172,9 @@ size_t legendre_coef_bytes(const LegendreTables& t, int nmaps) {
   const size_t vec = sizeof(double2) * (size_t)(std::max<long>(t.nlm, 1) + 2 * kSynCH) * padded_maps(nmaps) *
                      (t.spin ? 2 : 1);
   if (nmaps < 3) return vec;
-  // matrix-core layout: 32 doubles per chain per block of 4 l, 2 blocks of padding (prefetch)
-  const size_t mf = sizeof(double) * 32 * (t.spin ? 2 : 1) * (size_t)(t.nblk + 2);
+  // matrix-core layout: 32 doubles per chain, map group and block of 4 l, 2 blocks of padding (prefetch)
+  const int ns = nmaps >= kLegendreMfma8Min ? 2 : 1;
+  const size_t mf = sizeof(double) * 32 * ns * (t.spin ? 2 : 1) * (size_t)(t.nblk + 2);
   return std::max(vec, mf);
 }
 
@@ -1164,14 +1209,21 @@ static void legendre_synth_chunk(const LegendreTables& t, int nmaps, int ncomp, 
                                  long alm_stride_map, const double* fl, double2* coef_ws, double2* G, long sm, long sr,
                                  long smap, hipStream_t st);
 
-static void legendre_synth_mfma(const LegendreTables& t, int nmaps, int ncomp, const AlmPtrs4& alm,
+static void legendre_synth_mfma(const LegendreTables& t, int nmaps, int ncomp, const AlmPtrs8& alm,
                                 const double* fl, double2* coef_ws, double2* G, long sm, long sr, long smap,
                                 hipStream_t st);
 
-static AlmPtrs4 strided_maps(const double2* alm, long stride, int nmaps) {
-  AlmPtrs4 a{};
-  for (int b = 0; b < 4; ++b) a.p[b] = alm + (b < nmaps ? b : 0) * stride;
+static AlmPtrs8 strided_maps(const double2* alm, long stride, int nmaps) {
+  AlmPtrs8 a{};
+  for (int b = 0; b < 8; ++b) a.p[b] = alm + (b < nmaps ? b : 0) * stride;
   return a;
+}
+
+// maps of the next matrix-core launch for `left` remaining maps (>= 3); NUSHT_LEGENDRE_MFMA8=0 keeps 4 per launch
+static int mfma_maps(int left) {
+  const char* e = getenv("NUSHT_LEGENDRE_MFMA8");
+  const bool eight = !(e && e[0] == '0');
+  return eight && left >= kLegendreMfma8Min ? std::min(8, left) : std::min(kLegendreMfmaMaps, left);
 }
 
 void legendre_synth(const LegendreTables& t, int nmaps, int ncomp, const double2* alm, long alm_stride_map,
@@ -1182,7 +1234,7 @@ void legendre_synth(const LegendreTables& t, int nmaps, int ncomp, const double2
   while (b < nmaps) {
     const int left = nmaps - b;
     if (t.mfma && left >= 3) {
-      const int nb = std::min(kLegendreMfmaMaps, left);
+      const int nb = mfma_maps(left);
       legendre_synth_mfma(t, nb, ncomp, strided_maps(alm + b * alm_stride_map, alm_stride_map, nb), fl, coef_ws,
                           G + b * smap, sm, sr, smap, st);
       b += nb;
@@ -1202,9 +1254,9 @@ void legendre_synth_maps(const LegendreTables& t, int nmaps, int ncomp, const do
   while (b < nmaps) {
     const int left = nmaps - b;
     if (t.mfma && left >= 3) {
-      const int nb = std::min(kLegendreMfmaMaps, left);
-      AlmPtrs4 ap{};
-      for (int k = 0; k < 4; ++k) ap.p[k] = alm[b + (k < nb ? k : 0)];
+      const int nb = mfma_maps(left);
+      AlmPtrs8 ap{};
+      for (int k = 0; k < 8; ++k) ap.p[k] = alm[b + (k < nb ? k : 0)];
       legendre_synth_mfma(t, nb, ncomp, ap, nullptr, coef_ws, G + b * smap, sm, sr, smap, st);
       b += nb;
     } else {
@@ -1214,18 +1266,23 @@ void legendre_synth_maps(const LegendreTables& t, int nmaps, int ncomp, const do
   }
 }
 
-static void legendre_synth_mfma(const LegendreTables& t, int nmaps, int ncomp, const AlmPtrs4& alm,
+static void legendre_synth_mfma(const LegendreTables& t, int nmaps, int ncomp, const AlmPtrs8& alm,
                                 const double* fl, double2* coef_ws, double2* G, long sm, long sr, long smap,
                                 hipStream_t st) {
+  NUSHT_REQUIRE(nmaps >= 1 && nmaps <= 8, kErrArg, "matrix-core synthesis: 1 to 8 maps per launch");
   const int L = t.lmax;
+  const int ns = nmaps > kLegendreMfmaMaps ? 2 : 1;  // 16-column map groups
   double* coef4 = reinterpret_cast<double*>(coef_ws);
   {
     StageScope sc(t.n_prep.c_str(), st);
-    const int nc = t.spin ? 2 : 1;
     // block row y covers m = y and m = L - y; its x extent is sized for m = 0 (the longest table)
-    dim3 grid(ceil_div((L + 4) / 4 * 16, 256), L / 2 + 1);
-    hipLaunchKernelGGL(k_prep4, grid, dim3(256), 0, st, L, t.spin, nmaps, ncomp, alm, fl,
-                       t.mofs.as<long>(), t.mblk.as<long>(), t.cnorm.as<double>(), coef4);
+    dim3 grid(ceil_div((L + 4) / 4 * 16 * ns, 256), L / 2 + 1);
+    if (ns == 1)
+      hipLaunchKernelGGL(k_prep4<1>, grid, dim3(256), 0, st, L, t.spin, nmaps, ncomp, alm, fl, t.mofs.as<long>(),
+                         t.mblk.as<long>(), t.cnorm.as<double>(), coef4);
+    else
+      hipLaunchKernelGGL(k_prep4<2>, grid, dim3(256), 0, st, L, t.spin, nmaps, ncomp, alm, fl, t.mofs.as<long>(),
+                         t.mblk.as<long>(), t.cnorm.as<double>(), coef4);
     NUSHT_LAUNCH_CHECK();
   }
   SynthArgs a;
@@ -1244,10 +1301,14 @@ static void legendre_synth_mfma(const LegendreTables& t, int nmaps, int ncomp, c
   a.G = G; a.sm = sm; a.sr = sr; a.smap = smap;
   const int nblocks = a.nsb * ceil_div(L + 1, kNumXcd) * kNumXcd;
   StageScope sc(t.n_synth.c_str(), st);
-  if (t.spin > 0)
-    hipLaunchKernelGGL(k_synth_mfma<true>, dim3(nblocks), dim3(64), 0, st, a, coef4, t.mblk.as<long>());
+  if (t.spin > 0 && ns == 1)
+    hipLaunchKernelGGL((k_synth_mfma<true, 1>), dim3(nblocks), dim3(64), 0, st, a, coef4, t.mblk.as<long>());
+  else if (t.spin > 0)
+    hipLaunchKernelGGL((k_synth_mfma<true, 2>), dim3(nblocks), dim3(64), 0, st, a, coef4, t.mblk.as<long>());
+  else if (ns == 1)
+    hipLaunchKernelGGL((k_synth_mfma<false, 1>), dim3(nblocks), dim3(64), 0, st, a, coef4, t.mblk.as<long>());
   else
-    hipLaunchKernelGGL(k_synth_mfma<false>, dim3(nblocks), dim3(64), 0, st, a, coef4, t.mblk.as<long>());
+    hipLaunchKernelGGL((k_synth_mfma<false, 2>), dim3(nblocks), dim3(64), 0, st, a, coef4, t.mblk.as<long>());
   NUSHT_LAUNCH_CHECK();
 }
 

@@ -518,8 +518,8 @@ def test_batch_every_variant(spin, monkeypatch):
 @pytest.mark.parametrize("spin", [0, 2])
 @pytest.mark.parametrize("nmaps", [4, 5, 7, 8])
 def test_batch_matrix_core_path(spin, nmaps, monkeypatch):
-    """Batches of >= 3 maps run the Legendre contraction on the matrix cores (4 maps per launch; the
-    remainder on the vector kernels, or a second launch for 3).  Compared with the vector-only path
+    """Batches of >= 3 maps run the Legendre contraction on the matrix cores (4 maps per launch, 7 or 8 in one
+    launch of two column groups; the remainder on the vector kernels).  Compared with the vector-only path
     (NUSHT_LEGENDRE_MFMA=0) at lmax 200 (two groups of ring pairs, underflow near the poles), within 1e-12
     relative."""
     from cunusht_amd.transformer import NuPlan
